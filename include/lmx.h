@@ -1,0 +1,257 @@
+/*
+ * lmx.h -- C ABI of the MI355X-native Wilson(-clover) D_W / D^dagger D library.
+ *
+ * One context = one lattice geometry + one HIP stream.  Every entry point
+ * returns an int status (LMX_OK on success); lmx_last_error() gives the
+ * message of the last failure on the calling thread.  No C++ exception
+ * crosses the ABI.  All field pointers are DEVICE pointers; sizes are in
+ * bytes or counts as named.  Calls are asynchronous on the context stream
+ * unless documented otherwise (solvers and reductions return host values and
+ * are therefore synchronous).  A context is not thread-safe: use one per host
+ * thread.
+ *
+ * Internal (device) layouts, sites x-fastest (site = x + Lx*(y + Ly*(z + Lz*t))):
+ *   spinor  F64: double2 [nrhs][12][V]    component k = spin*3 + colour
+ *           F32: float2  [nrhs][12][V]
+ *   links   F64: double2 [4][9][V]        element k = row*3 + col
+ *           F32: float2  [4][9][V]    F16: half2 [4][9][V]
+ *   clover  F64: double2 [27][V]          6 anti-Hermitian planes (12,13,14,23,24,34):
+ *           slots 0..8 hold the 18 imaginary diagonals (plane*3+i) in pairs,
+ *           slots 9..26 the upper off-diagonals (plane*3 + {01,02,12}).
+ * The reference layout (Julia, column-major, halo width nw) is
+ *   spinor  A[3,4,L1+2nw,...,L4+2nw]   complex, colour fastest
+ *   link    A[3,3,L1+2nw,...,L4+2nw]   per direction, row fastest
+ * and is converted by lmx_spinor_import/export and lmx_links_import.
+ *
+ * Reference citations are relative to the LatticeMatrices.jl source tree.
+ */
+#ifndef LMX_H
+#define LMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMX_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    LMX_OK = 0,
+    LMX_ERR_ARG = 1,         /* bad argument (ArgumentError in the reference) */
+    LMX_ERR_ALIAS = 2,       /* fields that must be distinct alias each other */
+    LMX_ERR_HIP = 3,         /* HIP runtime failure */
+    LMX_ERR_NOMEM = 4,       /* device allocation failed */
+    LMX_ERR_UNSUPPORTED = 5, /* combination not built */
+    LMX_ERR_WORKSPACE = 6    /* caller workspace too small */
+};
+
+/* ---- precisions of stored fields --------------------------------------- */
+enum { LMX_F64 = 0, LMX_F32 = 1, LMX_F16 = 2 };
+
+/* ---- operator flags ------------------------------------------------------ */
+enum {
+    LMX_DAGGER = 1, /* apply D^dagger (Adjoint_WilsonDiracOperator4D) */
+    LMX_DONLY = 2   /* hopping term only, coefficient +1/2 (WilsonDiracOperator4D_Donly) */
+};
+
+/* ---- CG reasons: CGResult.reason, src/LinearAlgebras/cg.jl:1-13 --------- */
+enum {
+    LMX_CG_CONVERGED = 0,
+    LMX_CG_MAXITER = 1,
+    LMX_CG_NONPOSITIVE_CURVATURE = 2,
+    LMX_CG_BREAKDOWN = 3
+};
+
+typedef struct lmx_context lmx_context;
+typedef struct lmx_wilson lmx_wilson;
+
+/* Solver parameters.  Defaults (lmx_cg_params_default) are cg!'s keyword
+ * defaults rtol=1e-10, atol=0, maxiter=5000 (src/LinearAlgebras/cg.jl:73-74). */
+typedef struct {
+    double rtol;
+    double atol;
+    int32_t maxiter;
+    int32_t precision;        /* inner precision of lmx_cg_batched: LMX_F64 (plain), LMX_F32, LMX_F16 links */
+    double reliable_delta;    /* reliable update when |r_inner| < delta * |r at last update| */
+    int32_t reliable_maxiter; /* ... or after this many inner iterations */
+    int32_t check_every;      /* iterations between host convergence polls */
+    int32_t use_graph;        /* capture check_every iterations in a hipGraph */
+    int32_t rhs_block;        /* right-hand sides per dslash launch (0: autotuned) */
+    int32_t reserved[4];
+} lmx_cg_params;
+
+/* CGResult (src/LinearAlgebras/cg.jl:7-13) plus the true residual and the
+ * reliable-update count of the mixed solver. */
+typedef struct {
+    int32_t converged;
+    int32_t iterations;
+    int32_t reason;
+    int32_t reliable_updates;
+    double residual_norm;
+    double relative_residual;
+    double true_relative_residual; /* |b - A x| / |b| in FP64, or -1 if not computed */
+} lmx_cg_result;
+
+/* Caller-side operator for lmx_cg_generic: out = A in, enqueued on `stream`.
+ * Returns 0 on success. */
+typedef int (*lmx_apply_fn)(void* user, void* out, const void* in, void* stream);
+
+/* ======================================================================= */
+/* Context                                                                  */
+/* ======================================================================= */
+
+/* Last error message of the calling thread ("" if none). */
+const char* lmx_last_error(void);
+int lmx_abi_version(void);
+
+/* Create a context for a lattice of extents dims (x,y,z,t).  phases holds the
+ * spinor boundary phases as (re, im) pairs per direction; NULL means all ones.
+ * They play the role of LatticeMatrix(...; phases) on the spinor lattice
+ * (src/LatticeMatrices_core.jl:297-307,161).  Zero phases are rejected, as in
+ * the reference (:329-330).  stream: hipStream_t to run on (NULL: the null
+ * stream).  Replaces the single-rank construction of LatticeMatrix /
+ * LatticeMatrix_standard (src/LatticeMatrices_core.jl:297-307). */
+int lmx_context_create(lmx_context** ctx, const int32_t dims[4], const double* phases, void* stream);
+int lmx_context_destroy(lmx_context* ctx);
+int lmx_context_set_stream(lmx_context* ctx, void* stream);
+int lmx_context_synchronize(lmx_context* ctx);
+int64_t lmx_context_volume(const lmx_context* ctx);
+size_t lmx_spinor_bytes(const lmx_context* ctx, int precision, int nrhs);
+size_t lmx_links_bytes(const lmx_context* ctx, int precision);
+size_t lmx_clover_bytes(const lmx_context* ctx, int precision);
+
+/* ======================================================================= */
+/* Layout boundary                                                          */
+/* ======================================================================= */
+
+/* Reference padded AoS complex128 spinor(s) -> internal F64 SoA.  src holds
+ * nrhs consecutive arrays A[3,4,L1+2nw,...] (LatticeMatrix.A,
+ * src/LatticeMatrices_core.jl:361-363); only the core is read (halo ignored). */
+int lmx_spinor_import(lmx_context* ctx, void* dst, const void* src_padded, int nw, int nrhs);
+/* Internal F64 SoA -> reference padded AoS; the halo is filled with the
+ * phase-wrapped core values exactly as set_halo! would (:707-723). */
+int lmx_spinor_export(lmx_context* ctx, void* dst_padded, int nw, const void* src, int nrhs);
+/* Four reference link arrays A[3,3,L+2nw...] (complex128) -> internal F64 links. */
+int lmx_links_import(lmx_context* ctx, void* dst, const void* const src_padded[4], int nw);
+/* Internal F64 links -> four padded link arrays (halo periodic). */
+int lmx_links_export(lmx_context* ctx, void* const dst_padded[4], int nw, const void* src);
+
+/* ======================================================================= */
+/* BLAS-1 on spinor fields (F64, nrhs fields stacked)                        */
+/* ======================================================================= */
+
+/* Y = a*X + b*Y   -- axpby!, src/LinearAlgebras/linearalgebra_D.jl:2-23 */
+int lmx_axpby(lmx_context* ctx, double a, const void* X, double b, void* Y, int nrhs);
+/* out = sum conj(A) B (complex) -- dot, src/LinearAlgebras/linearalgebra_D.jl:3325-3341.
+ * Synchronous; out[0]=re, out[1]=im. */
+int lmx_dot(lmx_context* ctx, const void* A, const void* B, double out[2]);
+/* out = s * in (complex scalar) -- mul!(C, a::Number, x), src/LinearAlgebras/mul.jl:2-9 */
+int lmx_scale(lmx_context* ctx, void* out, const void* in, double s_re, double s_im, int nrhs);
+
+/* ======================================================================= */
+/* Wilson and Wilson-clover operators                                       */
+/* ======================================================================= */
+
+/* Create a Wilson operator over internal F64 links (borrowed, not copied;
+ * the caller keeps them alive) with hopping parameter kappa.  csw != 0 makes
+ * it the Wilson-clover operator and builds the field-strength cache.
+ * Replaces WilsonDiracOperator4D(U, kappa) (src/Operators/WilsonDiracOperator.jl:2-35)
+ * and WilsonDiracCloverOperator4D(U, kappa, cSW)
+ * (src/Operators/WilsonDiracCloverOperator.jl:447-456). */
+int lmx_wilson_create(lmx_context* ctx, lmx_wilson** op, const void* links, double kappa, double csw);
+int lmx_wilson_destroy(lmx_wilson* op);
+int lmx_wilson_set_kappa(lmx_wilson* op, double kappa);
+/* Point the operator at (possibly new) links and refresh every derived copy
+ * (low-precision links, clover cache) -- update_clover!(operator, U),
+ * src/Operators/WilsonDiracCloverOperator.jl:468-479. */
+int lmx_wilson_update_links(lmx_wilson* op, const void* links);
+/* Export the six clover planes as full F64 3x3 matrices, [6][V][3][3]
+ * complex, planes (12,13,14,23,24,34) -- CloverFieldStrength4D components. */
+int lmx_clover_export(lmx_wilson* op, void* F_out);
+
+/* out = D in  (flags: LMX_DAGGER, LMX_DONLY) on nrhs stacked F64 fields.
+ * Replaces mul!(C, D, psi), mul!(C, D', psi) and the _Donly variants
+ * (src/Operators/WilsonDiracOperator.jl:60-76,425-441,535-551,646-662) and the
+ * clover mul! (src/Operators/WilsonDiracCloverOperator.jl:561-601). */
+int lmx_wilson_apply(lmx_wilson* op, void* out, const void* in, int flags, int nrhs);
+/* out = D^dagger D in with caller temp -- DdagDOp(D, temp) mul!,
+ * src/Operators/DiracOperators.jl:104-121 (temp must not alias in/out). */
+int lmx_ddagd_apply(lmx_wilson* op, void* out, const void* in, void* tmp, int nrhs);
+
+/* ======================================================================= */
+/* Solvers                                                                  */
+/* ======================================================================= */
+
+void lmx_cg_params_default(lmx_cg_params* p);
+
+/* cg!(x, DdagDOp(D, tmp), b, r, p, Ap; rtol, atol, maxiter) in FP64 with the
+ * reference's operation order (src/LinearAlgebras/cg.jl:73-137).  x is the
+ * initial guess and the solution; r, p, Ap, tmp are caller-owned work fields
+ * whose contents are unspecified on return.  Fields must not alias
+ * (LMX_ERR_ALIAS, as _validate_cg_arguments :33-47). */
+int lmx_cg_ddagd(lmx_wilson* op, void* x, const void* b, void* r, void* p, void* Ap, void* tmp,
+                 const lmx_cg_params* params, lmx_cg_result* result);
+
+/* cg! with a caller operator A (callback on the context stream), FP64,
+ * single field -- cg!(x, A, b, r, p, Ap) for any mul!-implementing A. */
+int lmx_cg_generic(lmx_context* ctx, lmx_apply_fn A, void* A_user, void* x, const void* b, void* r, void* p,
+                   void* Ap, const lmx_cg_params* params, lmx_cg_result* result);
+
+/* Batched D^dagger D solve of nrhs right-hand sides (x, b: nrhs stacked F64
+ * fields, x is the initial guess).  params->precision selects plain FP64 CG
+ * per RHS (LMX_F64) or the mixed solver: FP32 (LMX_F32) or FP16-stored links
+ * with FP32 spinors (LMX_F16) inner CG with FP64 reliable updates, FP64
+ * reductions and FP64 solution accumulation.  Convergence is on the FP64
+ * true residual |b - A x| <= max(atol, rtol |b|) per RHS.  The workspace is
+ * caller-owned (size from lmx_cg_batched_workspace).  results[nrhs]. */
+int lmx_cg_batched_workspace(lmx_wilson* op, int nrhs, const lmx_cg_params* params, size_t* bytes);
+int lmx_cg_batched(lmx_wilson* op, void* x, const void* b, int nrhs, void* workspace, size_t workspace_bytes,
+                   const lmx_cg_params* params, lmx_cg_result* results);
+
+/* ======================================================================= */
+/* Force                                                                    */
+/* ======================================================================= */
+
+/* Link cotangent of the Wilson mul!: dU[mu](x) += coefficient * (...) for
+ * dresult and psi (F64 spinors) -- the Enzyme reverse rule kernel
+ * _kernel_wilson_link_pullback! (ext/AD/WilsonDiracOperator.jl:62-91,137-143)
+ * with _wilson_halfspin_link_pullback_direction3!
+ * (src/Operators/WilsonDiracOperator.jl:354-410).  dU: internal F64 links layout. */
+int lmx_wilson_pullback(lmx_wilson* op, void* dU, const void* dresult, const void* psi, double coefficient);
+
+/* F[mu](x) = TA(U[mu](x) G[mu](x)^dagger) (kernel_traceless_antihermitian!,
+ * src/TA/TA.jl:68-116); all in internal F64 links layout. */
+int lmx_ta_project(lmx_context* ctx, void* F, const void* links, const void* G);
+
+/* dSFdU(dfdU, D, phi) (ext/LatticeMatricesEnzymeExt.jl:395-460) for the
+ * Wilson operator: eta = (D^dagger D)^-1 phi by FP64 CG from zero, chi = D eta,
+ * G = pullback(dresult = -2 chi, eta, coefficient -kappa) (the Enzyme
+ * convention G = dS/dReU + i dS/dImU), and, if F is non-NULL, the su(3) force
+ * F = TA(U G^dagger).  eta/chi: caller F64 spinors (outputs).  work: 4 spinors. */
+int lmx_wilson_force(lmx_wilson* op, void* G, void* F, const void* phi, void* eta, void* chi, void* work,
+                     const lmx_cg_params* params, lmx_cg_result* result);
+
+/* ======================================================================= */
+/* Autotuning                                                               */
+/* ======================================================================= */
+
+typedef struct {
+    int32_t rhs_block;   /* chosen RHS block */
+    int32_t block_size;  /* threads per workgroup */
+    double ms_per_apply; /* measured D apply time of the chosen shape, per RHS */
+} lmx_tune_result;
+
+/* Time the D^dagger D inner dslash of `precision` over RHS blocks
+ * {1,2,4,6,12} (those dividing nrhs first) and workgroup sizes, cache the
+ * winner per (volume, precision, nrhs, arch) in cache_path (may be NULL) and
+ * use it for later lmx_cg_batched calls with rhs_block = 0. */
+int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char* cache_path, lmx_tune_result* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LMX_H */

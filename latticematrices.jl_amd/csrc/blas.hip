@@ -1,0 +1,293 @@
+// blas.hip -- BLAS-1 and the fused FP64 CG vector kernels.
+//
+// Each reduction is one pass: every block reduces its 256 sites with the
+// fixed tree of lmx_device.h, publishes a partial, and the last block to
+// finish sums the partials in a fixed order and updates the device-resident
+// CG state (alpha, beta, convergence) without a host round trip.  The
+// element-wise updates use the reference's unfused forms
+//   axpby!(a, X, b, Y): Y = a*X + b*Y   src/LinearAlgebras/linearalgebra_D.jl:15-23
+// and the per-site dot kernel_dot_D (:3331-3341).
+#include "lmx_internal.h"
+
+namespace lmx {
+
+enum { FIN_OUT = 0, FIN_BB = 1, FIN_RR0 = 2, FIN_RRNEW = 3, FIN_PAP = 4 };
+
+__device__ __forceinline__ void finalize_scalar(int kind, CGState* st, double re, double im, double* out2) {
+    switch (kind) {
+    case FIN_OUT:
+        out2[0] = re;
+        out2[1] = im;
+        break;
+    case FIN_BB: {  // cg.jl:78-86
+        st->bb = re;
+        if (!isfinite(re) || re < 0.0) {
+            st->done = 1;
+            st->reason = LMX_CG_BREAKDOWN;
+            st->rnorm = NAN;
+            st->bnorm = NAN;
+        } else {
+            st->bnorm = sqrt(re);
+            st->tol = fmax(st->atol, st->rtol * st->bnorm);
+        }
+        break;
+    }
+    case FIN_RR0: {  // cg.jl:95-103
+        st->rr = re;
+        if (!isfinite(re) || re < 0.0) {
+            st->done = 1;
+            st->reason = LMX_CG_BREAKDOWN;
+            st->rnorm = NAN;
+        } else {
+            st->rnorm = sqrt(re);
+            if (st->rnorm <= st->tol) {
+                st->done = 1;
+                st->converged = 1;
+                st->reason = LMX_CG_CONVERGED;
+            } else if (st->maxiter <= 0) {
+                st->done = 1;
+                st->reason = LMX_CG_MAXITER;
+            }
+        }
+        break;
+    }
+    case FIN_PAP: {  // cg.jl:107-116
+        st->pAp = re;
+        if (!isfinite(re)) {
+            st->done = 1;
+            st->reason = LMX_CG_BREAKDOWN;
+        } else if (re <= 0.0) {
+            st->done = 1;
+            st->reason = LMX_CG_NONPOSITIVE_CURVATURE;
+        } else {
+            st->alpha = st->rr / re;
+        }
+        break;
+    }
+    case FIN_RRNEW: {  // cg.jl:120-132
+        st->rr_new = re;
+        st->iteration += 1;
+        if (!isfinite(re) || re < 0.0) {
+            st->done = 1;
+            st->reason = LMX_CG_BREAKDOWN;
+            st->rnorm = NAN;
+        } else {
+            st->rnorm = sqrt(re);
+            if (st->rnorm <= st->tol) {
+                st->done = 1;
+                st->converged = 1;
+                st->reason = LMX_CG_CONVERGED;
+            } else if (st->iteration >= st->maxiter) {
+                st->done = 1;
+                st->reason = LMX_CG_MAXITER;
+            } else {
+                st->beta = re / st->rr;
+                st->rr = re;
+            }
+        }
+        break;
+    }
+    }
+}
+
+// Shared tail: reduce (re, im) over the grid and finalize in the last block.
+__device__ __forceinline__ void grid_reduce(double re, double im, bool want_im, int lb, double* partials,
+                                            unsigned* counter, int kind, CGState* st, double* out2) {
+    __shared__ double lds4[4];
+    __shared__ int flag;
+    const int nb = gridDim.x;
+    double tre = block_tree(re, lds4);
+    double tim = want_im ? block_tree(im, lds4) : 0.0;
+    if (threadIdx.x == 0) {
+        partials[lb] = tre;
+        if (want_im) partials[nb + lb] = tim;
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        flag = (prev == (unsigned)nb - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (flag) {
+        __threadfence();
+        double fre = finish_partials(partials, nb, lds4);
+        double fim = want_im ? finish_partials(partials + nb, nb, lds4) : 0.0;
+        if (threadIdx.x == 0) {
+            finalize_scalar(kind, st, fre, fim, out2);
+            *counter = 0u;
+        }
+    }
+}
+
+// dot(A, B) = sum conj(A) B
+__global__ void __launch_bounds__(256) k_dot(const double2* __restrict__ A, const double2* __restrict__ B, int V,
+                                             double* partials, unsigned* counter, int kind, CGState* st,
+                                             double* out2) {
+    if (st != nullptr && st->done) return;
+    const int lb = blockIdx.x;
+    const int s = lb * 256 + threadIdx.x;
+    double re = 0.0, im = 0.0;
+    if (s < V) {
+        cx<double> acc = mk<double>(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            double2 a = A[(long)k * V + s], b = B[(long)k * V + s];
+            acc = cadd(acc, cmul(cconj(mk<double>(a.x, a.y)), mk<double>(b.x, b.y)));
+        }
+        re = acc.re;
+        im = acc.im;
+    }
+    grid_reduce(re, im, kind == FIN_OUT, lb, partials, counter, kind, st, out2);
+}
+
+__global__ void __launch_bounds__(256) k_axpby(double a, const double* __restrict__ X, double b, double* __restrict__ Y,
+                                               int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) Y[i] = a * X[i] + b * Y[i];
+}
+
+__global__ void __launch_bounds__(256) k_scale(double2* __restrict__ out, const double2* __restrict__ in, double sr,
+                                               double si, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        double2 v = in[i];
+        cx<double> z = cmul(mk<double>(sr, si), mk<double>(v.x, v.y));
+        out[i] = make_double2(z.re, z.im);
+    }
+}
+
+// r = b - Ap (axpby!(1,b,0,r); axpby!(-1,Ap,1,r)); p = r; rr = |r|^2  (cg.jl:91-95)
+__global__ void __launch_bounds__(256) k_cg_setup_r(const double2* __restrict__ b, const double2* __restrict__ Ap,
+                                                    double2* __restrict__ r, double2* __restrict__ p, int V,
+                                                    double* partials, unsigned* counter, CGState* st) {
+    if (st->done) return;
+    const int lb = blockIdx.x;
+    const int s = lb * 256 + threadIdx.x;
+    double re = 0.0;
+    if (s < V) {
+        cx<double> acc = mk<double>(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const long i = (long)k * V + s;
+            double2 bv = b[i], av = Ap[i];
+            double2 rv = make_double2(-1.0 * av.x + bv.x, -1.0 * av.y + bv.y);
+            r[i] = rv;
+            p[i] = rv;
+            cx<double> rc = mk<double>(rv.x, rv.y);
+            acc = cadd(acc, cmul(cconj(rc), rc));
+        }
+        re = acc.re;
+    }
+    grid_reduce(re, 0.0, false, lb, partials, counter, FIN_RR0, st, nullptr);
+}
+
+// x = alpha*p + x; r = (-alpha)*Ap + r; rr_new = |r|^2  (cg.jl:116-120)
+__global__ void __launch_bounds__(256) k_cg_update_xr(double2* __restrict__ x, double2* __restrict__ r,
+                                                      const double2* __restrict__ p, const double2* __restrict__ Ap,
+                                                      int V, double* partials, unsigned* counter, CGState* st) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const double malpha = -alpha;
+    const int lb = blockIdx.x;
+    const int s = lb * 256 + threadIdx.x;
+    double re = 0.0;
+    if (s < V) {
+        cx<double> acc = mk<double>(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const long i = (long)k * V + s;
+            double2 pv = p[i], xv = x[i], av = Ap[i], rv = r[i];
+            xv = make_double2(alpha * pv.x + 1.0 * xv.x, alpha * pv.y + 1.0 * xv.y);
+            rv = make_double2(malpha * av.x + 1.0 * rv.x, malpha * av.y + 1.0 * rv.y);
+            x[i] = xv;
+            r[i] = rv;
+            cx<double> rc = mk<double>(rv.x, rv.y);
+            acc = cadd(acc, cmul(cconj(rc), rc));
+        }
+        re = acc.re;
+    }
+    grid_reduce(re, 0.0, false, lb, partials, counter, FIN_RRNEW, st, nullptr);
+}
+
+// p = 1*r + beta*p  (cg.jl:130-131)
+__global__ void __launch_bounds__(256) k_cg_update_p(const double2* __restrict__ r, double2* __restrict__ p, int V,
+                                                     CGState* st) {
+    if (st->done) return;
+    const double beta = st->beta;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s < V) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const long i = (long)k * V + s;
+            double2 rv = r[i], pv = p[i];
+            p[i] = make_double2(1.0 * rv.x + beta * pv.x, 1.0 * rv.y + beta * pv.y);
+        }
+    }
+}
+
+__global__ void k_cg_init_state(CGState* st, double rtol, double atol, int maxiter) {
+    if (threadIdx.x == 0) {
+        CGState s;
+        s.bb = s.bnorm = s.tol = s.rr = s.pAp = s.alpha = s.beta = s.rr_new = s.rnorm = 0.0;
+        s.rtol = rtol;
+        s.atol = atol;
+        s.iteration = 0;
+        s.maxiter = maxiter;
+        s.done = 0;
+        s.reason = LMX_CG_MAXITER;
+        s.converged = 0;
+        s.pad[0] = s.pad[1] = s.pad[2] = 0;
+        *st = s;
+    }
+}
+
+static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+int launch_dot(lmx_context* ctx, const double2* A, const double2* B, int fin_kind, CGState* st, double* out2) {
+    const int nb = nblocks(ctx->V);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(256), 0, ctx->stream, A, B, (int)ctx->V, ctx->partials,
+                       ctx->counters + 0, fin_kind, st, out2);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+int launch_axpby(lmx_context* ctx, double a, const double2* X, double b, double2* Y, int64_t n_complex) {
+    const int64_t n = 2 * n_complex;
+    if (n == 0) return LMX_OK;
+    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(256), 0, ctx->stream, a, (const double*)X, b, (double*)Y, n);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+int launch_scale(lmx_context* ctx, double2* out, const double2* in, double sr, double si, int64_t n) {
+    if (n == 0) return LMX_OK;
+    hipLaunchKernelGGL(k_scale, grid_for(n), dim3(256), 0, ctx->stream, out, in, sr, si, n);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+int launch_cg_setup_r(lmx_context* ctx, const double2* b, const double2* Ap, double2* r, double2* p, CGState* st) {
+    hipLaunchKernelGGL(k_cg_setup_r, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, b, Ap, r, p, (int)ctx->V,
+                       ctx->partials, ctx->counters + 0, st);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+int launch_cg_update_xr(lmx_context* ctx, double2* x, double2* r, const double2* p, const double2* Ap, CGState* st) {
+    hipLaunchKernelGGL(k_cg_update_xr, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, x, r, p, Ap, (int)ctx->V,
+                       ctx->partials, ctx->counters + 0, st);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+int launch_cg_update_p(lmx_context* ctx, const double2* r, double2* p, CGState* st) {
+    hipLaunchKernelGGL(k_cg_update_p, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, r, p, (int)ctx->V, st);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+int launch_cg_init_state(lmx_context* ctx, CGState* st, double rtol, double atol, int maxiter) {
+    hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(64), 0, ctx->stream, st, rtol, atol, maxiter);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+}  // namespace lmx

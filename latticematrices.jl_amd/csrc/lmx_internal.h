@@ -1,0 +1,133 @@
+// lmx_internal.h -- host-side structures shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/lmx.h"
+#include "lmx_device.h"
+
+namespace lmx {
+
+// Thread-local last error (lmx_last_error).
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define LMX_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            ::lmx::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return LMX_ERR_HIP;                                                                    \
+        }                                                                                          \
+    } while (0)
+
+#define LMX_CHECK(cond, code, ...)         \
+    do {                                   \
+        if (!(cond)) {                     \
+            ::lmx::set_error(__VA_ARGS__); \
+            return (code);                 \
+        }                                  \
+    } while (0)
+
+#define LMX_TRY(call)              \
+    do {                           \
+        int rc_ = (call);          \
+        if (rc_ != LMX_OK) return rc_; \
+    } while (0)
+
+// Device-resident state of one CG solve (single RHS).  Written only by the
+// finishing block of each reduction; read by every kernel of the iteration
+// for its early exit.
+struct CGState {
+    double bb, bnorm, tol, rr, pAp, alpha, beta, rr_new, rnorm;
+    double rtol, atol;
+    int iteration, maxiter, done, reason, converged;
+    int pad[3];
+};
+
+}  // namespace lmx
+
+struct lmx_context {
+    lmx::Geom g;
+    int64_t V;
+    hipStream_t stream;
+    int device;
+    // Reduction scratch: per-block partials and self-resetting completion counters.
+    double* partials;
+    size_t partials_cap;  // doubles
+    unsigned* counters;   // kCounters entries
+    lmx::CGState* state;  // device CG states (kMaxStates)
+    lmx::CGState* host_state;  // pinned mirror
+    double* dot_out;      // device scalars for lmx_dot
+    double* host_dot;     // pinned mirror
+};
+
+struct lmx_wilson {
+    lmx_context* ctx;
+    const double2* links;  // borrowed F64 links
+    double kappa;
+    double csw;
+    // Owned derived copies.
+    float2* links32;
+    __half2* links16;
+    double2* clover64;
+    float2* clover32;
+    __half2* clover16;
+    bool lo_valid;
+    // Autotuned launch shape per (precision, nrhs); 0 = untuned.
+    int tuned_block[3][13];
+};
+
+namespace lmx {
+
+constexpr int kCounters = 64;
+constexpr int kMaxStates = 64;
+
+inline int nblocks(int64_t V) { return int((V + kRedBlock - 1) / kRedBlock); }
+
+// ---- launchers (wilson.hip) ----
+struct Epilogue {
+    int kind;           // 0 none, 1 pAp (p = aux, finalize alpha)
+    const double2* aux; // p field for kind 1
+    CGState* st;        // early-exit state (may be null)
+    double* partials;
+    unsigned* counter;
+};
+
+int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
+                      double2* out, const double2* in, int nrhs, int flags, const Epilogue& epi);
+
+int launch_wilson_lo(lmx_context* ctx, int link_prec, const void* U, const void* clover, double kappa, double csw,
+                     float2* out, const float2* in, int nrhs, int flags, int rhs_block, CGState* st);
+
+int launch_clover_build_f64(lmx_context* ctx, const double2* U, double2* clover);
+int launch_clover_export(lmx_context* ctx, const double2* clover, double2* F_out);
+int launch_convert_links(lmx_context* ctx, const double2* U, float2* U32, __half2* U16);
+int launch_convert_clover(lmx_context* ctx, const double2* F, float2* F32, __half2* F16);
+
+// ---- BLAS / CG kernels (blas.hip) ----
+int launch_dot(lmx_context* ctx, const double2* A, const double2* B, int fin_kind, CGState* st, double* out2);
+int launch_axpby(lmx_context* ctx, double a, const double2* X, double b, double2* Y, int64_t n);
+int launch_scale(lmx_context* ctx, double2* out, const double2* in, double sr, double si, int64_t n);
+int launch_cg_setup_r(lmx_context* ctx, const double2* b, const double2* Ap, double2* r, double2* p, CGState* st);
+int launch_cg_update_xr(lmx_context* ctx, double2* x, double2* r, const double2* p, const double2* Ap, CGState* st);
+int launch_cg_update_p(lmx_context* ctx, const double2* r, double2* p, CGState* st);
+int launch_cg_init_state(lmx_context* ctx, CGState* st, double rtol, double atol, int maxiter);
+
+// ---- pullback / TA (force.hip) ----
+int launch_pullback_f64(lmx_context* ctx, double2* dU, const double2* dres, const double2* psi, double coefficient);
+int launch_ta_project(lmx_context* ctx, double2* F, const double2* U, const double2* G);
+
+// ---- batched / mixed CG (mixed.hip) ----
+int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void* work, size_t bytes,
+                    const lmx_cg_params* params, lmx_cg_result* results);
+size_t cg_batched_workspace_impl(lmx_wilson* op, int nrhs, const lmx_cg_params* params);
+
+int ensure_lo_links(lmx_wilson* op);
+
+}  // namespace lmx

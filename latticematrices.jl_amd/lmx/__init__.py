@@ -1,0 +1,19 @@
+"""lmx: MI355X-native Wilson(-clover) D_W / D^dagger D CG path of LatticeMatrices.jl.
+
+The compute path is liblmx.so (hand-written gfx950 HIP kernels behind the C
+ABI of include/lmx.h); this package is its host-side mirror of the
+reference's LatticeMatrix operator API.
+"""
+from . import _lib, generators
+from ._lib import ArgumentError, LmxError
+from .lattice import (Adjoint, CGResult, DdagDOp, GaugeField, Lattice, ScaledOperator, SpinorField,
+                      WilsonDiracCloverOperator4D, WilsonDiracOperator4D, WilsonDiracOperator4D_Donly, adjoint,
+                      axpby_, cg_, clear_matrix_, dot, dSFdU, mul_, pseudofermion_action, solve_, substitute_,
+                      ta_project_, wilson_pullback_)
+
+__all__ = [
+    "ArgumentError", "LmxError", "Adjoint", "CGResult", "DdagDOp", "GaugeField", "Lattice", "ScaledOperator",
+    "SpinorField", "WilsonDiracCloverOperator4D", "WilsonDiracOperator4D", "WilsonDiracOperator4D_Donly",
+    "adjoint", "axpby_", "cg_", "clear_matrix_", "dot", "dSFdU", "mul_", "pseudofermion_action", "solve_",
+    "substitute_", "ta_project_", "wilson_pullback_", "generators",
+]

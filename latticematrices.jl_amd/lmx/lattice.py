@@ -1,0 +1,453 @@
+"""Host-side mirror of the reference's LatticeMatrix operator API for the
+Wilson(-clover) D^dagger D path, backed by liblmx.so.
+
+Reference interface (LatticeMatrices.jl) -> this module:
+  LatticeMatrix(NC1, NC2, dim, gsize, PEs; phases)  -> Lattice + SpinorField / GaugeField
+  WilsonDiracOperator4D(U, kappa)                   -> WilsonDiracOperator4D(U, kappa)
+  WilsonDiracOperator4D_Donly(U)                    -> WilsonDiracOperator4D_Donly(U)
+  WilsonDiracCloverOperator4D(U, kappa, cSW)        -> WilsonDiracCloverOperator4D(U, kappa, cSW)
+  adjoint(D) / D'                                   -> adjoint(D)
+  mul!(C, A, psi)                                   -> mul_(C, A, psi)
+  DdagDOp(D, temp)                                  -> DdagDOp(D, temp)
+  cg!(x, A, b, r, p, Ap; rtol, atol, maxiter)       -> cg_(x, A, b, r, p, Ap, rtol=, atol=, maxiter=)
+  solve!(x, A::DdagDOp, b, r, p, Ap; kw...)         -> solve_(...)
+  pseudofermion_action(D, phi, eta, Deta, r, p, Ap) -> pseudofermion_action(...)
+  dSFdU(dfdU, D, phi)                               -> dSFdU(dfdU, D, phi)
+  axpby!, dot, clear_matrix!, substitute!           -> axpby_, dot, clear_matrix_, substitute_
+Errors: ArgumentError for bad geometry, aliasing and tolerances (as the
+reference); CG non-convergence is a CGResult status, not an exception.
+
+Fields live on the GPU (torch tensors used as device memory) in the
+library's structure-of-arrays layout; numpy arrays in the reference's
+column-major layout ([V,4,3] spinors, [4,V,3,3] links, optionally with a halo
+of width nw) are converted on the device by the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from numbers import Number
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, check, lib
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
+
+
+def _require_gpu():
+    if torch is None or not torch.cuda.is_available():
+        raise RuntimeError("liblmx requires a ROCm GPU (torch.cuda.is_available() is False)")
+
+
+def _ptr(t) -> int:
+    return t.data_ptr()
+
+
+class Lattice:
+    """Geometry, spinor boundary phases and the HIP stream of one lattice
+    (the single-rank LatticeMatrix construction, src/LatticeMatrices_core.jl:297-343)."""
+
+    def __init__(self, dims, phases=(1, 1, 1, 1), device=None):
+        _require_gpu()
+        dims = tuple(int(d) for d in dims)
+        if len(dims) != 4:
+            raise ArgumentError(_lib.LMX_ERR_ARG, f"global size must have 4 entries, got {len(dims)}")
+        if len(phases) != 4:
+            raise ArgumentError(_lib.LMX_ERR_ARG, f"phases must have 4 entries, got {len(phases)}")
+        self.dims = dims
+        self.phases = tuple(complex(p) for p in phases)
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        self.V = int(np.prod(dims))
+        with torch.cuda.device(self.device):
+            self.stream = torch.cuda.current_stream(self.device)
+        ph = (ctypes.c_double * 8)(*[v for p in self.phases for v in (p.real, p.imag)])
+        d = (ctypes.c_int32 * 4)(*dims)
+        h = ctypes.c_void_p()
+        check(lib().lmx_context_create(ctypes.byref(h), d, ph, ctypes.c_void_p(self.stream.cuda_stream)))
+        self._ctx = h
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def synchronize(self):
+        check(lib().lmx_context_synchronize(self._ctx))
+
+    def __del__(self):
+        try:
+            if getattr(self, "_ctx", None):
+                lib().lmx_context_destroy(self._ctx)
+                self._ctx = None
+        except Exception:
+            pass
+
+    # -- field factories -------------------------------------------------
+    def spinor(self, nrhs: int = 1) -> "SpinorField":
+        return SpinorField(self, nrhs)
+
+    def spinor_from_numpy(self, arr, nw: int = 0) -> "SpinorField":
+        return SpinorField.from_numpy(self, arr, nw)
+
+    def gauge_from_numpy(self, U, nw: int = 0) -> "GaugeField":
+        return GaugeField.from_numpy(self, U, nw)
+
+
+class SpinorField:
+    """nrhs Dirac spinor fields (3 colours x 4 spins per site), FP64, in the
+    device layout double2[nrhs][12][V]."""
+
+    def __init__(self, lattice: Lattice, nrhs: int = 1, data=None):
+        self.lattice = lattice
+        self.nrhs = int(nrhs)
+        if data is None:
+            data = torch.zeros((self.nrhs, 12, lattice.V, 2), dtype=torch.float64, device=lattice.device)
+        self.data = data
+
+    @property
+    def ptr(self):
+        return ctypes.c_void_p(_ptr(self.data))
+
+    def similar(self) -> "SpinorField":
+        return SpinorField(self.lattice, self.nrhs)
+
+    @classmethod
+    def from_numpy(cls, lattice: Lattice, arr, nw: int = 0) -> "SpinorField":
+        """arr: complex [V,4,3] / [nrhs,V,4,3] (nw = 0) or the padded reference
+        array(s) [nrhs, Lt+2nw, Lz+2nw, Ly+2nw, Lx+2nw, 4, 3] in C order."""
+        a = np.ascontiguousarray(arr, dtype=np.complex128)
+        per = int(np.prod([d + 2 * nw for d in lattice.dims])) * 12
+        if a.size % per != 0:
+            raise ArgumentError(_lib.LMX_ERR_ARG, f"array of {a.size} elements is not a multiple of {per}")
+        nrhs = a.size // per
+        f = cls(lattice, nrhs)
+        src = torch.from_numpy(a.reshape(-1)).to(lattice.device)
+        check(lib().lmx_spinor_import(lattice.ctx, f.ptr, ctypes.c_void_p(_ptr(src)), int(nw), nrhs))
+        lattice.synchronize()
+        return f
+
+    def to_padded(self, nw: int = 0):
+        L = self.lattice
+        n = self.nrhs * int(np.prod([d + 2 * nw for d in L.dims])) * 12
+        dst = torch.empty(n, dtype=torch.complex128, device=L.device)
+        check(lib().lmx_spinor_export(L.ctx, ctypes.c_void_p(_ptr(dst)), int(nw), self.ptr, self.nrhs))
+        L.synchronize()
+        return dst.cpu().numpy()
+
+    def to_numpy(self):
+        """[V,4,3] (nrhs == 1) or [nrhs,V,4,3] complex128."""
+        a = self.to_padded(0).reshape(self.nrhs, self.lattice.V, 4, 3)
+        return a[0] if self.nrhs == 1 else a
+
+    def rhs(self, i: int) -> "SpinorField":
+        """View of right-hand side i as a single field."""
+        return SpinorField(self.lattice, 1, self.data[i:i + 1])
+
+
+class GaugeField:
+    """The four link fields U_1..U_4 (SU(3)), FP64, device layout double2[4][9][V]."""
+
+    def __init__(self, lattice: Lattice, data=None):
+        self.lattice = lattice
+        if data is None:
+            data = torch.zeros((4, 9, lattice.V, 2), dtype=torch.float64, device=lattice.device)
+        self.data = data
+
+    @property
+    def ptr(self):
+        return ctypes.c_void_p(_ptr(self.data))
+
+    @classmethod
+    def from_numpy(cls, lattice: Lattice, U, nw: int = 0) -> "GaugeField":
+        """U: complex [4, V, 3, 3] (mu, site, row, col) with nw = 0, or four padded
+        reference link arrays (any container of 4 arrays of V_padded*9 values in
+        the reference's column-major element order when nw > 0)."""
+        g = cls(lattice)
+        if nw == 0 and not isinstance(U, (list, tuple)):
+            a = np.ascontiguousarray(U, dtype=np.complex128).reshape(4, lattice.V, 3, 3)
+            # reference element order within a site is column-major (row fastest)
+            arrs = [np.ascontiguousarray(np.swapaxes(a[mu], -1, -2)).reshape(-1) for mu in range(4)]
+        else:
+            arrs = [np.ascontiguousarray(u, dtype=np.complex128).reshape(-1) for u in U]
+        dev = [torch.from_numpy(x).to(lattice.device) for x in arrs]
+        ptrs = (ctypes.c_void_p * 4)(*[_ptr(t) for t in dev])
+        check(lib().lmx_links_import(lattice.ctx, g.ptr, ptrs, int(nw)))
+        lattice.synchronize()
+        return g
+
+    def to_numpy(self):
+        """[4, V, 3, 3] complex128 (mu, site, row, col)."""
+        L = self.lattice
+        dev = [torch.empty(L.V * 9, dtype=torch.complex128, device=L.device) for _ in range(4)]
+        ptrs = (ctypes.c_void_p * 4)(*[_ptr(t) for t in dev])
+        check(lib().lmx_links_export(L.ctx, ptrs, 0, self.ptr))
+        L.synchronize()
+        out = np.stack([t.cpu().numpy().reshape(L.V, 3, 3) for t in dev])
+        return np.ascontiguousarray(np.swapaxes(out, -1, -2))
+
+
+# ---------------------------------------------------------------------------
+# Operators
+# ---------------------------------------------------------------------------
+class WilsonDiracOperator4D:
+    """D = 1 - kappa sum_mu [U_mu(x)(1-g_mu) psi(x+mu) + U_mu(x-mu)^dagger (1+g_mu) psi(x-mu)]
+    (src/Operators/WilsonDiracOperator.jl:2-35,57-76).  csw != 0 gives the
+    Wilson-clover operator (WilsonDiracCloverOperator4D)."""
+
+    def __init__(self, U: GaugeField, kappa: float, csw: float = 0.0):
+        self.lattice = U.lattice
+        self.U = U
+        self.csw = float(csw)
+        h = ctypes.c_void_p()
+        check(lib().lmx_wilson_create(self.lattice.ctx, ctypes.byref(h), U.ptr, float(kappa), float(csw)))
+        self._op = h
+        self._kappa = float(kappa)
+
+    @property
+    def kappa(self):
+        return self._kappa
+
+    @kappa.setter
+    def kappa(self, value):
+        check(lib().lmx_wilson_set_kappa(self._op, float(value)))
+        self._kappa = float(value)
+
+    def update_links(self, U: GaugeField | None = None):
+        """update_clover!(operator[, U]) (src/Operators/WilsonDiracCloverOperator.jl:468-479)."""
+        if U is not None:
+            self.U = U
+        check(lib().lmx_wilson_update_links(self._op, self.U.ptr))
+
+    def clover_planes(self):
+        """The six clover field-strength planes [6, V, 3, 3] (CloverFieldStrength4D)."""
+        L = self.lattice
+        out = torch.empty(6 * L.V * 9, dtype=torch.complex128, device=L.device)
+        check(lib().lmx_clover_export(self._op, ctypes.c_void_p(_ptr(out))))
+        L.synchronize()
+        return out.cpu().numpy().reshape(6, L.V, 3, 3)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_op", None):
+                lib().lmx_wilson_destroy(self._op)
+                self._op = None
+        except Exception:
+            pass
+
+
+def WilsonDiracCloverOperator4D(U: GaugeField, kappa: float, cSW: float) -> WilsonDiracOperator4D:
+    """D = D_W - kappa cSW sum_{mu<nu} gamma_mu gamma_nu F_mu,nu
+    (src/Operators/WilsonDiracCloverOperator.jl:403-456)."""
+    return WilsonDiracOperator4D(U, kappa, cSW)
+
+
+class WilsonDiracOperator4D_Donly:
+    """(1/2) sum_mu [U(1-g)psi(x+mu) + U^dagger(1+g)psi(x-mu)] (:522-551)."""
+
+    def __init__(self, U: GaugeField):
+        self._w = WilsonDiracOperator4D(U, 0.0)
+        self.U = U
+        self.lattice = U.lattice
+
+
+@dataclass
+class Adjoint:
+    parent: object
+
+
+def adjoint(A):
+    """adjoint(D) is involutive; adjoint(DdagDOp) is itself (DiracOperators.jl:102)."""
+    if isinstance(A, Adjoint):
+        return A.parent
+    if isinstance(A, DdagDOp):
+        return A
+    return Adjoint(A)
+
+
+@dataclass
+class DdagDOp:
+    """Allocation-free normal operator D^dagger D with a caller-owned temp
+    (src/Operators/DiracOperators.jl:93-121)."""
+    D: object
+    temp: SpinorField
+
+
+def _wilson_handle(A):
+    if isinstance(A, WilsonDiracOperator4D):
+        return A._op, 0
+    if isinstance(A, WilsonDiracOperator4D_Donly):
+        return A._w._op, _lib.LMX_DONLY
+    if isinstance(A, Adjoint):
+        h, f = _wilson_handle(A.parent)
+        return h, f | _lib.LMX_DAGGER
+    raise TypeError(f"not a Wilson operator: {type(A).__name__}")
+
+
+def mul_(C: SpinorField, A, psi: SpinorField) -> SpinorField:
+    """mul!(C, A, psi) for the Wilson family, DdagDOp, or a complex scalar
+    (mul!(C, a::Number, x), src/LinearAlgebras/mul.jl:2-9)."""
+    if C.nrhs != psi.nrhs:
+        raise ArgumentError(_lib.LMX_ERR_ARG, "C and psi must hold the same number of fields")
+    if isinstance(A, Number):
+        a = complex(A)
+        check(lib().lmx_scale(C.lattice.ctx, C.ptr, psi.ptr, a.real, a.imag, C.nrhs))
+        return C
+    if isinstance(A, DdagDOp):
+        if A.temp is psi or A.temp is C:
+            raise ArgumentError(_lib.LMX_ERR_ALIAS, "DdagDOp temporary field must not alias its input or output")
+        if isinstance(A.D, WilsonDiracOperator4D):
+            check(lib().lmx_ddagd_apply(A.D._op, C.ptr, psi.ptr, A.temp.ptr, C.nrhs))
+            return C
+        mul_(A.temp, A.D, psi)
+        mul_(C, adjoint(A.D), A.temp)
+        return C
+    if isinstance(A, ScaledOperator):
+        return mul_(C, A.scale, psi)
+    if isinstance(A, Adjoint) and isinstance(A.parent, ScaledOperator):
+        return mul_(C, complex(A.parent.scale).conjugate(), psi)
+    h, flags = _wilson_handle(A)
+    check(lib().lmx_wilson_apply(h, C.ptr, psi.ptr, flags, C.nrhs))
+    return C
+
+
+@dataclass
+class ScaledOperator:
+    """A = scale * I with a device apply (the _CGScaledOperator of test/cg.jl:1-10)."""
+    scale: complex
+
+
+# ---------------------------------------------------------------------------
+# BLAS-1
+# ---------------------------------------------------------------------------
+def axpby_(a: float, X: SpinorField, b: float, Y: SpinorField) -> SpinorField:
+    """Y = a X + b Y (src/LinearAlgebras/linearalgebra_D.jl:2-23)."""
+    check(lib().lmx_axpby(Y.lattice.ctx, float(a), X.ptr, float(b), Y.ptr, Y.nrhs))
+    return Y
+
+
+def dot(A: SpinorField, B: SpinorField) -> complex:
+    """sum conj(A) B (src/LinearAlgebras/linearalgebra_D.jl:3325-3341)."""
+    out = (ctypes.c_double * 2)()
+    check(lib().lmx_dot(A.lattice.ctx, A.ptr, B.ptr, out))
+    return complex(out[0], out[1])
+
+
+def clear_matrix_(A: SpinorField) -> SpinorField:
+    A.data.zero_()
+    return A
+
+
+def substitute_(A: SpinorField, B: SpinorField) -> SpinorField:
+    A.data.copy_(B.data)
+    return A
+
+
+# ---------------------------------------------------------------------------
+# Solvers
+# ---------------------------------------------------------------------------
+@dataclass
+class CGResult:
+    """CGResult (src/LinearAlgebras/cg.jl:7-13) + true residual / reliable updates."""
+    converged: bool
+    iterations: int
+    residual_norm: float
+    relative_residual: float
+    reason: str
+    true_relative_residual: float = -1.0
+    reliable_updates: int = 0
+
+    @classmethod
+    def from_c(cls, r):
+        return cls(bool(r.converged), int(r.iterations), float(r.residual_norm), float(r.relative_residual),
+                   _lib.REASONS[int(r.reason)], float(r.true_relative_residual), int(r.reliable_updates))
+
+
+def _aliases(fields):
+    names = ("x", "b", "r", "p", "Ap")
+    for i in range(len(fields)):
+        for j in range(i + 1, len(fields)):
+            if fields[i] is fields[j] or fields[i].data.data_ptr() == fields[j].data.data_ptr():
+                raise ArgumentError(_lib.LMX_ERR_ALIAS, f"CG fields {names[i]} and {names[j]} must not alias")
+
+
+def cg_(x, A, b, r, p, Ap, rtol=1e-10, atol=0.0, maxiter=5000, check_every=0) -> CGResult:
+    """cg!(x, A, b, r, p, Ap; rtol, atol, maxiter) (src/LinearAlgebras/cg.jl:73-137)."""
+    _aliases((x, b, r, p, Ap))
+    params = _lib.default_params(rtol=float(rtol), atol=float(atol), maxiter=int(maxiter),
+                                 check_every=int(check_every))
+    if int(maxiter) < 0:
+        raise ArgumentError(_lib.LMX_ERR_ARG, "maxiter must be nonnegative")
+    res = _lib.CGResultC()
+    if isinstance(A, DdagDOp) and isinstance(A.D, WilsonDiracOperator4D):
+        check(lib().lmx_cg_ddagd(A.D._op, x.ptr, b.ptr, r.ptr, p.ptr, Ap.ptr, A.temp.ptr, ctypes.byref(params),
+                                 ctypes.byref(res)))
+        return CGResult.from_c(res)
+    lattice = x.lattice
+    shapes = {f.data.data_ptr(): f.data for f in (x, b, r, p, Ap)}
+
+    def apply(user, out, inp, stream):
+        try:
+            o = SpinorField(lattice, 1, _wrap_ptr(out, lattice, shapes))
+            i = SpinorField(lattice, 1, _wrap_ptr(inp, lattice, shapes))
+            mul_(o, A, i)
+            return 0
+        except Exception:  # pragma: no cover - reported as a status
+            return 1
+
+    cb = _lib.APPLY_FN(apply)
+    check(lib().lmx_cg_generic(lattice.ctx, cb, None, x.ptr, b.ptr, r.ptr, p.ptr, Ap.ptr, ctypes.byref(params),
+                               ctypes.byref(res)))
+    return CGResult.from_c(res)
+
+
+def _wrap_ptr(ptr, lattice, cache):
+    """Recover the SpinorField tensor for a device pointer handed back by the
+    generic CG callback (one of the caller's five fields)."""
+    return cache[ptr]
+
+
+def solve_(x, A: DdagDOp, b, r, p, Ap, **kw) -> CGResult:
+    """solve!(x, A::DdagDOp, b, r, p, Ap; kwargs...) (DiracOperators.jl:130-132)."""
+    return cg_(x, A, b, r, p, Ap, **kw)
+
+
+def pseudofermion_action(D, phi, eta, Deta, r, p, Ap, **kw) -> float:
+    """real(dot(phi, (D^dagger D)^-1 phi)); eta is the initial guess and the
+    solution (src/Operators/DiracOperators.jl:155-162)."""
+    res = solve_(eta, DdagDOp(D, Deta), phi, r, p, Ap, **kw)
+    if not res.converged:
+        raise RuntimeError(f"CG failed with reason {res.reason} after {res.iterations} iterations; "
+                           f"relative residual = {res.relative_residual}")
+    return dot(phi, eta).real
+
+
+def dSFdU(dfdU: GaugeField, D: WilsonDiracOperator4D, phi: SpinorField, force: GaugeField | None = None,
+          rtol=1e-10, maxiter=5000):
+    """dSFdU(dfdU, D, phi) (ext/LatticeMatricesEnzymeExt.jl:395-460): dfdU <-
+    dS/dU of S = phi^dagger (D^dagger D)^-1 phi in the Enzyme convention; with
+    `force`, also the su(3) force TA(U dfdU^dagger).  Returns (CGResult, eta, chi)."""
+    L = D.lattice
+    eta, chi = SpinorField(L), SpinorField(L)
+    work = torch.empty((4, 12, L.V, 2), dtype=torch.float64, device=L.device)
+    params = _lib.default_params(rtol=float(rtol), maxiter=int(maxiter))
+    res = _lib.CGResultC()
+    check(lib().lmx_wilson_force(D._op, dfdU.ptr, force.ptr if force is not None else None, phi.ptr, eta.ptr,
+                                 chi.ptr, ctypes.c_void_p(_ptr(work)), ctypes.byref(params), ctypes.byref(res)))
+    return CGResult.from_c(res), eta, chi
+
+
+def wilson_pullback_(dU: GaugeField, D: WilsonDiracOperator4D, dresult: SpinorField, psi: SpinorField,
+                     coefficient: float) -> GaugeField:
+    """The Enzyme reverse-rule link cotangent of mul!(result, D, psi)
+    (ext/AD/WilsonDiracOperator.jl:62-91,137-143): dU += coefficient * ..."""
+    check(lib().lmx_wilson_pullback(D._op, dU.ptr, dresult.ptr, psi.ptr, float(coefficient)))
+    return dU
+
+
+def ta_project_(F: GaugeField, U: GaugeField, G: GaugeField) -> GaugeField:
+    check(lib().lmx_ta_project(U.lattice.ctx, F.ptr, U.ptr, G.ptr))
+    return F

@@ -1,0 +1,793 @@
+/*
+ * lmx_oracle.c -- CPU oracle for the Wilson(-clover) D_W / D_W^dagger path and
+ * the D^dagger D conjugate-gradient solve of LatticeMatrices.jl.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or, for cpu_baseline, as the timed CPU restatement).  The product library
+ * (liblmx.so) never links, loads or calls it.
+ *
+ * Every routine restates the reference's arithmetic in the reference's
+ * operation order (file:line cited per function).  Conventions:
+ *   - Complex numbers are pairs of doubles (re, im).
+ *   - Julia's `muladd(a::Float64, b, c)` is taken to be one fused multiply-add
+ *     (LLVM contracts it on every FMA-capable host); the complex `muladd` is
+ *     Julia Base's decomposition
+ *         re = muladd(re z, re w, -muladd(im z, im w, -re x))
+ *         im = muladd(re z, im w,  muladd(im z, re w,  im x)).
+ *   - Plain complex `*`, `+`, `-` are not contracted (Julia does not contract
+ *     without muladd/@fastmath); this file is compiled with -ffp-contract=off.
+ *   - Spinor layout: psi[(site*4 + spin)*3 + colour] complex, site index
+ *     x-fastest (src/Latticeindices.jl:88-95).  This is the reference's
+ *     (3,4,X,Y,Z,T) column-major array without halo.
+ *   - Link layout: U[((mu*V + site)*3 + row)*3 + col] complex (row-major 3x3).
+ *   - Boundary phases act on spinors only: a hop that wraps forward in
+ *     direction mu multiplies the neighbour spinor by phase[mu], a backward
+ *     wrap by inv(phase[mu]) (src/LatticeMatrices_core.jl:707-723,1066-1070).
+ *     Applying the phase while reading the neighbour is bit-identical to the
+ *     reference's halo fill (the same complex product of the same values).
+ *
+ * Parity pinning (tests/test_oracle_*.py):
+ *   wilson       : dense gamma-matrix restatement of test/wilson_halfspin.jl:8-42
+ *                  (numpy), free-field plane-wave closed form, adjoint identity,
+ *                  gamma5-hermiticity, gauge covariance, tiny-lattice brute force.
+ *   dot/axpby/cg : test/cg.jl known answers (scaled operator, zero rhs,
+ *                  maxiter 0, indefinite operator, D^dagger D of (2+i)I),
+ *                  S = eta^dagger eta for phi = D^dagger eta, true residual.
+ *   clover       : dense restatement of test/wilson_clover.jl:47-112, csw=0
+ *                  equals Wilson, unit gauge gives F=0, F anti-Hermitian,
+ *                  gamma5-hermiticity, gauge covariance.
+ *   pullback/TA  : central finite differences of S along su(3) directions,
+ *                  linear-functional finite differences of <chi, D(U) psi>.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    double re, im;
+} cx;
+
+static inline cx C(double r, double i) {
+    cx z;
+    z.re = r;
+    z.im = i;
+    return z;
+}
+static inline cx cadd(cx a, cx b) { return C(a.re + b.re, a.im + b.im); }
+static inline cx csub(cx a, cx b) { return C(a.re - b.re, a.im - b.im); }
+static inline cx cneg(cx a) { return C(-a.re, -a.im); }
+static inline cx cconj(cx a) { return C(a.re, -a.im); }
+/* Julia Base complex product (no contraction). */
+static inline cx cmul(cx a, cx b) {
+    return C(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+/* Float64 * Complex */
+static inline cx rmul(double a, cx b) { return C(a * b.re, a * b.im); }
+/* Julia Base complex muladd with fused real muladds. */
+static inline cx cmuladd(cx z, cx w, cx x) {
+    return C(fma(z.re, w.re, -fma(z.im, w.im, -x.re)), fma(z.re, w.im, fma(z.im, w.re, x.im)));
+}
+/* src/Operators/WilsonDiracOperator.jl:149-155 */
+static inline cx muladdmulti(cx a1, cx b1, cx a2, cx b2, cx a3, cx b3) {
+    cx acc = C(0.0, 0.0);
+    acc = cmuladd(a1, b1, acc);
+    acc = cmuladd(a2, b2, acc);
+    acc = cmuladd(a3, b3, acc);
+    return acc;
+}
+/* PM * im * z with PM = +-1 (exact). */
+static inline cx pmi(int pm, cx z) { return C(-(double)pm * z.im, (double)pm * z.re); }
+static inline cx pmr(int pm, cx z) { return C((double)pm * z.re, (double)pm * z.im); }
+
+typedef struct {
+    int L[4];
+    long V;
+    long stride[4];
+    cx ph[4];
+    cx iph[4];
+} geom;
+
+/* Complex reciprocal used for backward boundary phases (inv(phase)). */
+static cx cinv(cx z) {
+    double d = z.re * z.re + z.im * z.im;
+    return C(z.re / d, -z.im / d);
+}
+
+static void geom_init(geom* g, const int* L, const double* phases) {
+    g->V = 1;
+    for (int d = 0; d < 4; ++d) {
+        g->L[d] = L[d];
+        g->stride[d] = g->V;
+        g->V *= L[d];
+        g->ph[d] = C(phases ? phases[2 * d] : 1.0, phases ? phases[2 * d + 1] : 0.0);
+        g->iph[d] = cinv(g->ph[d]);
+    }
+}
+
+static inline void site_coords(const geom* g, long s, int x[4]) {
+    for (int d = 0; d < 4; ++d) {
+        x[d] = (int)(s % g->L[d]);
+        s /= g->L[d];
+    }
+}
+
+/* neighbour site in direction mu, dir = +1/-1; *wrap = 1 when crossing the boundary */
+static inline long nbr(const geom* g, long s, const int x[4], int mu, int dir, int* wrap) {
+    if (dir > 0) {
+        if (x[mu] == g->L[mu] - 1) {
+            *wrap = 1;
+            return s - (long)(g->L[mu] - 1) * g->stride[mu];
+        }
+        *wrap = 0;
+        return s + g->stride[mu];
+    }
+    if (x[mu] == 0) {
+        *wrap = 1;
+        return s + (long)(g->L[mu] - 1) * g->stride[mu];
+    }
+    *wrap = 0;
+    return s - g->stride[mu];
+}
+
+static inline void load_spinor(const double* psi, long s, cx out[12]) {
+    const double* p = psi + s * 24;
+    for (int k = 0; k < 12; ++k) out[k] = C(p[2 * k], p[2 * k + 1]);
+}
+/* psi(spin, colour) at k = spin*3 + colour */
+#define SP(a, spin, col) (a)[(spin) * 3 + (col)]
+
+static inline void load_link(const double* U, const geom* g, int mu, long s, cx u[9]) {
+    const double* p = U + ((long)mu * g->V + s) * 18;
+    for (int k = 0; k < 9; ++k) u[k] = C(p[2 * k], p[2 * k + 1]);
+}
+
+/* _wilson_half_project3, src/Operators/WilsonDiracOperator.jl:157-175 */
+static inline void half_project(const cx psi[12], int col, int pm, int mu, cx* h1, cx* h2) {
+    cx p1 = SP(psi, 0, col), p2 = SP(psi, 1, col), p3 = SP(psi, 2, col), p4 = SP(psi, 3, col);
+    switch (mu) {
+    case 0:
+        *h1 = csub(p1, pmi(pm, p4));
+        *h2 = csub(p2, pmi(pm, p3));
+        break;
+    case 1:
+        *h1 = csub(p1, pmr(pm, p4));
+        *h2 = cadd(p2, pmr(pm, p3));
+        break;
+    case 2:
+        *h1 = csub(p1, pmi(pm, p3));
+        *h2 = cadd(p2, pmi(pm, p4));
+        break;
+    default:
+        *h1 = csub(p1, pmr(pm, p3));
+        *h2 = csub(p2, pmr(pm, p4));
+        break;
+    }
+}
+
+/* _wilson_reconstruct_add3, src/Operators/WilsonDiracOperator.jl:231-263.
+ * acc is indexed [colour*4 + spin] (the reference accumulator order c11,c12,...). */
+static inline void reconstruct_add(cx acc[12], const cx h[6], int pm, int mu) {
+    for (int c = 0; c < 3; ++c) {
+        cx h1 = h[2 * c], h2 = h[2 * c + 1];
+        cx* a = acc + 4 * c;
+        a[0] = cadd(a[0], h1);
+        a[1] = cadd(a[1], h2);
+        switch (mu) {
+        case 0:
+            a[2] = cadd(a[2], pmi(pm, h2));
+            a[3] = cadd(a[3], pmi(pm, h1));
+            break;
+        case 1:
+            a[2] = cadd(a[2], pmr(pm, h2));
+            a[3] = csub(a[3], pmr(pm, h1));
+            break;
+        case 2:
+            a[2] = cadd(a[2], pmi(pm, h1));
+            a[3] = csub(a[3], pmi(pm, h2));
+            break;
+        default:
+            a[2] = csub(a[2], pmr(pm, h1));
+            a[3] = csub(a[3], pmr(pm, h2));
+            break;
+        }
+    }
+}
+
+static inline void scale_spinor(cx psi[12], cx ph) {
+    for (int k = 0; k < 12; ++k) psi[k] = cmul(psi[k], ph);
+}
+
+/* _wilson_hopping_accumulator3, src/Operators/WilsonDiracOperator.jl:265-320 */
+static void hopping_site(const geom* g, const double* U, const double* psi, long s, int fpm, cx acc[12]) {
+    int x[4];
+    site_coords(g, s, x);
+    for (int k = 0; k < 12; ++k) acc[k] = C(0.0, 0.0);
+    for (int mu = 0; mu < 4; ++mu) {
+        cx sp[12], u[9], h[6], r[6];
+        int wrap;
+        /* forward: U_mu(x) (1 + fpm gamma_mu... ) psi(x+mu)  (_wilson_half_matvec_forward3 :177-202) */
+        long sp_site = nbr(g, s, x, mu, +1, &wrap);
+        load_spinor(psi, sp_site, sp);
+        if (wrap) scale_spinor(sp, g->ph[mu]);
+        for (int c = 0; c < 3; ++c) half_project(sp, c, fpm, mu, &h[2 * c], &h[2 * c + 1]);
+        load_link(U, g, mu, s, u);
+        for (int a = 0; a < 3; ++a) {
+            r[2 * a] = muladdmulti(u[3 * a], h[0], u[3 * a + 1], h[2], u[3 * a + 2], h[4]);
+            r[2 * a + 1] = muladdmulti(u[3 * a], h[1], u[3 * a + 1], h[3], u[3 * a + 2], h[5]);
+        }
+        reconstruct_add(acc, r, fpm, mu);
+        /* backward: U_mu(x-mu)^dagger psi(x-mu)  (_wilson_half_matvec_backward3 :204-229) */
+        long sm_site = nbr(g, s, x, mu, -1, &wrap);
+        load_spinor(psi, sm_site, sp);
+        if (wrap) scale_spinor(sp, g->iph[mu]);
+        for (int c = 0; c < 3; ++c) half_project(sp, c, -fpm, mu, &h[2 * c], &h[2 * c + 1]);
+        load_link(U, g, mu, sm_site, u);
+        for (int a = 0; a < 3; ++a) {
+            cx w0 = cconj(u[a]), w1 = cconj(u[3 + a]), w2 = cconj(u[6 + a]);
+            r[2 * a] = muladdmulti(w0, h[0], w1, h[2], w2, h[4]);
+            r[2 * a + 1] = muladdmulti(w0, h[1], w1, h[3], w2, h[5]);
+        }
+        reconstruct_add(acc, r, -fpm, mu);
+    }
+}
+
+/*
+ * D_W, D_W^dagger and the hopping-only variants.
+ *   kernel_WilsonDiracOperator4D! NC=3            src/Operators/WilsonDiracOperator.jl:413-419
+ *   kernel_adjoint_WilsonDiracOperator4D! NC=3    :514-520
+ *   kernel_WilsonDiracOperator4D_Donly! NC=3      :624-631 (and adjoint :736-743)
+ *   _write_wilson_result3!                        :322-352
+ */
+void orc_wilson(double* out, const double* in, const double* U, const int* L, const double* phases,
+                double kappa, int dagger, int donly) {
+    geom g;
+    geom_init(&g, L, phases);
+    const int fpm = dagger ? 1 : -1;
+    const double coef = donly ? 0.5 : -kappa;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        cx acc[12];
+        hopping_site(&g, U, in, s, fpm, acc);
+        double* o = out + s * 24;
+        const double* p = in + s * 24;
+        for (int spin = 0; spin < 4; ++spin)
+            for (int c = 0; c < 3; ++c) {
+                cx h = acc[4 * c + spin];
+                cx base = donly ? C(0.0, 0.0) : C(p[2 * (spin * 3 + c)], p[2 * (spin * 3 + c) + 1]);
+                cx v = cadd(base, rmul(coef, h));
+                o[2 * (spin * 3 + c)] = v.re;
+                o[2 * (spin * 3 + c) + 1] = v.im;
+            }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* BLAS-1 and reductions                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* kernel_dot_D per-site value, src/LinearAlgebras/linearalgebra_D.jl:3331-3341:
+ * s = 0; for spin, for colour: s += conj(A)*B */
+static inline cx site_dot(const double* a, const double* b, long s) {
+    cx acc = C(0.0, 0.0);
+    const double* pa = a + s * 24;
+    const double* pb = b + s * 24;
+    for (int k = 0; k < 12; ++k) acc = cadd(acc, cmul(cconj(C(pa[2 * k], pa[2 * k + 1])), C(pb[2 * k], pb[2 * k + 1])));
+    return acc;
+}
+
+/* The fixed reduction tree of the HIP library: blocks of 256 sites, each wave
+ * of 64 halves its values (offsets 32..1), the 4 wave sums halve again; the
+ * block partials are summed by 256 strided sequential lanes which are then
+ * reduced by the same block tree.  Restated here so that the oracle's CG can
+ * reproduce the device's rounding exactly (order 1); order 0 is the plain
+ * sequential sum of a single-threaded reference run. */
+#define RBLK 256
+static void tree_halve(double* w, int n) {
+    for (int off = n / 2; off >= 1; off /= 2)
+        for (int i = 0; i < off; ++i) w[i] = w[i] + w[i + off];
+}
+static double block_tree(const double* v /*256*/) {
+    double waves[4];
+    double w[64];
+    for (int q = 0; q < 4; ++q) {
+        memcpy(w, v + 64 * q, sizeof(w));
+        tree_halve(w, 64);
+        waves[q] = w[0];
+    }
+    tree_halve(waves, 4);
+    return waves[0];
+}
+static double tree_finish(const double* partials, long nb) {
+    double lanes[RBLK];
+    for (int t = 0; t < RBLK; ++t) {
+        double acc = 0.0;
+        for (long b = t; b < nb; b += RBLK) acc = acc + partials[b];
+        lanes[t] = acc;
+    }
+    return block_tree(lanes);
+}
+
+typedef cx (*site_fn)(const void* ctx, long s);
+
+static cx reduce_sites(long V, int order, site_fn f, const void* ctx) {
+    if (order == 0) {
+        cx acc = C(0.0, 0.0);
+        for (long s = 0; s < V; ++s) acc = cadd(acc, f(ctx, s));
+        return acc;
+    }
+    long nb = (V + RBLK - 1) / RBLK;
+    double* pre = (double*)malloc(sizeof(double) * nb);
+    double* pim = (double*)malloc(sizeof(double) * nb);
+#pragma omp parallel for schedule(static)
+    for (long b = 0; b < nb; ++b) {
+        double vr[RBLK], vi[RBLK];
+        for (int t = 0; t < RBLK; ++t) {
+            long s = b * RBLK + t;
+            cx z = s < V ? f(ctx, s) : C(0.0, 0.0);
+            vr[t] = z.re;
+            vi[t] = z.im;
+        }
+        pre[b] = block_tree(vr);
+        pim[b] = block_tree(vi);
+    }
+    cx out = C(tree_finish(pre, nb), tree_finish(pim, nb));
+    free(pre);
+    free(pim);
+    return out;
+}
+
+typedef struct {
+    const double *a, *b;
+} dot_ctx;
+static cx dot_site_fn(const void* c, long s) {
+    const dot_ctx* d = (const dot_ctx*)c;
+    return site_dot(d->a, d->b, s);
+}
+
+/* LinearAlgebra.dot, src/LinearAlgebras/linearalgebra_D.jl:3325-3329 (one rank) */
+void orc_dot(const double* a, const double* b, long V, int order, double* out2) {
+    dot_ctx d = {a, b};
+    cx z = reduce_sites(V, order, dot_site_fn, &d);
+    out2[0] = z.re;
+    out2[1] = z.im;
+}
+
+/* axpby!(a, X, b, Y): Y = a*X + b*Y, src/LinearAlgebras/linearalgebra_D.jl:2-23.
+ * a and b are real here (CG only passes reals). */
+void orc_axpby(double a, const double* X, double b, double* Y, long V) {
+    long n = V * 24;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) Y[i] = a * X[i] + b * Y[i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Clover                                                                   */
+/* ------------------------------------------------------------------------ */
+
+static const int plane_mu[6] = {0, 0, 0, 1, 1, 2};
+static const int plane_nu[6] = {1, 2, 3, 2, 3, 3};
+
+/* _clover_mul_matrix3, src/Operators/WilsonDiracCloverOperator.jl:137-149 (row-major 9-tuples) */
+static void mat_mul(const cx* A, const cx* B, cx* out) {
+    cx r[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r[3 * i + j] = muladdmulti(A[3 * i], B[j], A[3 * i + 1], B[3 + j], A[3 * i + 2], B[6 + j]);
+    memcpy(out, r, sizeof(r));
+}
+static void mat_dag(const cx* A, cx* out) {
+    cx r[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r[3 * i + j] = cconj(A[3 * j + i]);
+    memcpy(out, r, sizeof(r));
+}
+/* _clover_path_matrix3 :159-162 = ((A*B)*C)*D */
+static void path4(const cx* A, const cx* B, const cx* Cm, const cx* D, cx* out) {
+    cx t[9];
+    mat_mul(A, B, t);
+    mat_mul(t, Cm, t);
+    mat_mul(t, D, out);
+}
+
+static long shift_site(const geom* g, long s, int mu, int dir) {
+    int x[4], w;
+    site_coords(g, s, x);
+    return nbr(g, s, x, mu, dir, &w);
+}
+
+/* kernel_clover_field_strength_halo! NC=3 + _clover_store_antihermitian3!,
+ * src/Operators/WilsonDiracCloverOperator.jl:164-226; planes (12,13,14,23,24,34) :1.
+ * Links are periodic (their phases are one).  F layout: F[((plane*V + site)*3+row)*3+col]. */
+void orc_clover_field_strength(double* F, const double* U, const int* L) {
+    geom g;
+    geom_init(&g, L, NULL);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        for (int pl = 0; pl < 6; ++pl) {
+            int mu = plane_mu[pl], nu = plane_nu[pl];
+            long xpmu = shift_site(&g, s, mu, +1), xpnu = shift_site(&g, s, nu, +1);
+            long xmmu = shift_site(&g, s, mu, -1), xmnu = shift_site(&g, s, nu, -1);
+            long xpnu_mmu = shift_site(&g, xpnu, mu, -1);
+            long xmmu_mnu = shift_site(&g, xmmu, nu, -1);
+            long xpmu_mnu = shift_site(&g, xmnu, mu, +1);
+            cx a[9], b[9], c[9], d[9], q[9], t[9];
+            /* +mu, +nu, -mu, -nu */
+            load_link(U, &g, mu, s, a);
+            load_link(U, &g, nu, xpmu, b);
+            load_link(U, &g, mu, xpnu, c);
+            mat_dag(c, c);
+            load_link(U, &g, nu, s, d);
+            mat_dag(d, d);
+            path4(a, b, c, d, q);
+            /* +nu, -mu, -nu, +mu */
+            load_link(U, &g, nu, s, a);
+            load_link(U, &g, mu, xpnu_mmu, b);
+            mat_dag(b, b);
+            load_link(U, &g, nu, xmmu, c);
+            mat_dag(c, c);
+            load_link(U, &g, mu, xmmu, d);
+            path4(a, b, c, d, t);
+            for (int k = 0; k < 9; ++k) q[k] = cadd(q[k], t[k]);
+            /* -mu, -nu, +mu, +nu */
+            load_link(U, &g, mu, xmmu, a);
+            mat_dag(a, a);
+            load_link(U, &g, nu, xmmu_mnu, b);
+            mat_dag(b, b);
+            load_link(U, &g, mu, xmmu_mnu, c);
+            load_link(U, &g, nu, xmnu, d);
+            path4(a, b, c, d, t);
+            for (int k = 0; k < 9; ++k) q[k] = cadd(q[k], t[k]);
+            /* -nu, +mu, +nu, -mu */
+            load_link(U, &g, nu, xmnu, a);
+            mat_dag(a, a);
+            load_link(U, &g, mu, xmnu, b);
+            load_link(U, &g, nu, xpmu_mnu, c);
+            load_link(U, &g, mu, s, d);
+            mat_dag(d, d);
+            path4(a, b, c, d, t);
+            for (int k = 0; k < 9; ++k) q[k] = cadd(q[k], t[k]);
+            /* store (q - q^dagger)/8 */
+            const double f = 1.0 / 8.0;
+            cx v11 = rmul(f, csub(q[0], cconj(q[0])));
+            cx v12 = rmul(f, csub(q[1], cconj(q[3])));
+            cx v13 = rmul(f, csub(q[2], cconj(q[6])));
+            cx v22 = rmul(f, csub(q[4], cconj(q[4])));
+            cx v23 = rmul(f, csub(q[5], cconj(q[7])));
+            cx v33 = rmul(f, csub(q[8], cconj(q[8])));
+            cx out[9] = {v11, v12, v13, cneg(cconj(v12)), v22, v23, cneg(cconj(v13)), cneg(cconj(v23)), v33};
+            double* o = F + ((long)pl * g.V + s) * 18;
+            for (int k = 0; k < 9; ++k) {
+                o[2 * k] = out[k].re;
+                o[2 * k + 1] = out[k].im;
+            }
+        }
+    }
+}
+
+static inline cx iunit(cx z) { return C(-z.im, z.re); } /* (0,1) * z */
+
+/* kernel_add_clover_term!, src/Operators/WilsonDiracCloverOperator.jl:481-525 */
+static void clover_site_add(const geom* g, const double* F, const double* psi, long s, double coefficient,
+                            cx res[12] /* [spin*3+colour] */) {
+    cx p[12];
+    load_spinor(psi, s, p);
+    for (int color = 0; color < 3; ++color) {
+        cx v1 = C(0, 0), v2 = C(0, 0), v3 = C(0, 0), v4 = C(0, 0);
+        for (int ic = 0; ic < 3; ++ic) {
+            cx f[6];
+            for (int pl = 0; pl < 6; ++pl) {
+                const double* q = F + (((long)pl * g->V + s) * 9 + color * 3 + ic) * 2;
+                f[pl] = C(q[0], q[1]);
+            }
+            cx f12 = f[0], f13 = f[1], f14 = f[2], f23 = f[3], f24 = f[4], f34 = f[5];
+            cx dp = iunit(cadd(f12, f34));
+            cx dm = iunit(csub(f12, f34));
+            cx odp = iunit(cadd(f14, f23));
+            cx odm = iunit(csub(f23, f14));
+            cx p1 = SP(p, 0, ic), p2 = SP(p, 1, ic), p3 = SP(p, 2, ic), p4 = SP(p, 3, ic);
+            v1 = cadd(v1, cadd(cmul(dp, p1), cmul(cadd(cadd(cneg(f13), odp), f24), p2)));
+            v2 = cadd(v2, cadd(cmul(cneg(dp), p2), cmul(csub(cadd(f13, odp), f24), p1)));
+            v3 = cadd(v3, cadd(cmul(dm, p3), cmul(csub(cadd(cneg(f13), odm), f24), p4)));
+            v4 = cadd(v4, cadd(cmul(cneg(dm), p4), cmul(cadd(cadd(f13, odm), f24), p3)));
+        }
+        SP(res, 0, color) = cadd(SP(res, 0, color), rmul(coefficient, v1));
+        SP(res, 1, color) = cadd(SP(res, 1, color), rmul(coefficient, v2));
+        SP(res, 2, color) = cadd(SP(res, 2, color), rmul(coefficient, v3));
+        SP(res, 3, color) = cadd(SP(res, 3, color), rmul(coefficient, v4));
+    }
+}
+
+/* Fused Wilson + clover, kernel_WilsonDiracCloverOperator4D! (:537-546) and its adjoint (:548-559):
+ * the Wilson result C = psi - kappa*hop is formed first, then the clover term is added with
+ * coefficient = -kappa * cSW. */
+void orc_wilson_clover(double* out, const double* in, const double* U, const double* F, const int* L,
+                       const double* phases, double kappa, double csw, int dagger) {
+    geom g;
+    geom_init(&g, L, phases);
+    const int fpm = dagger ? 1 : -1;
+    const double coefficient = (-kappa) * csw;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        cx acc[12], res[12];
+        hopping_site(&g, U, in, s, fpm, acc);
+        const double* p = in + s * 24;
+        for (int spin = 0; spin < 4; ++spin)
+            for (int c = 0; c < 3; ++c) {
+                cx base = C(p[2 * (spin * 3 + c)], p[2 * (spin * 3 + c) + 1]);
+                SP(res, spin, c) = cadd(base, rmul(-kappa, acc[4 * c + spin]));
+            }
+        clover_site_add(&g, F, in, s, coefficient, res);
+        double* o = out + s * 24;
+        for (int k = 0; k < 12; ++k) {
+            o[2 * k] = res[k].re;
+            o[2 * k + 1] = res[k].im;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CG                                                                       */
+/* ------------------------------------------------------------------------ */
+
+enum { ORC_CONVERGED = 0, ORC_MAXITER = 1, ORC_NONPOSITIVE_CURVATURE = 2, ORC_BREAKDOWN = 3 };
+
+enum { ORC_OP_DDAGD_WILSON = 0, ORC_OP_DDAGD_CLOVER = 1, ORC_OP_SCALE = 2, ORC_OP_DDAGD_SCALE = 3 };
+
+typedef struct {
+    int kind;
+    const double* U;
+    const double* F;
+    int L[4];
+    double phases[8];
+    double kappa, csw;
+    double scale_re, scale_im;
+    double* tmp; /* D x temporary (DdagDOp temp) */
+} orc_op;
+
+static void scale_field(double* out, const double* in, cx a, long V) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < V * 12; ++i) {
+        cx v = cmul(a, C(in[2 * i], in[2 * i + 1]));
+        out[2 * i] = v.re;
+        out[2 * i + 1] = v.im;
+    }
+}
+
+/* DdagDOp.mul!: temp = D x; y = D' temp (src/Operators/DiracOperators.jl:104-121) */
+static void op_apply(const orc_op* op, double* y, const double* x, long V) {
+    switch (op->kind) {
+    case ORC_OP_DDAGD_WILSON:
+        orc_wilson(op->tmp, x, op->U, op->L, op->phases, op->kappa, 0, 0);
+        orc_wilson(y, op->tmp, op->U, op->L, op->phases, op->kappa, 1, 0);
+        break;
+    case ORC_OP_DDAGD_CLOVER:
+        orc_wilson_clover(op->tmp, x, op->U, op->F, op->L, op->phases, op->kappa, op->csw, 0);
+        orc_wilson_clover(y, op->tmp, op->U, op->F, op->L, op->phases, op->kappa, op->csw, 1);
+        break;
+    case ORC_OP_SCALE: /* mul!(y, scale, x), src/LinearAlgebras/mul.jl:2-9 */
+        scale_field(y, x, C(op->scale_re, op->scale_im), V);
+        break;
+    default: /* D = scale, D' = conj(scale) */
+        scale_field(op->tmp, x, C(op->scale_re, op->scale_im), V);
+        scale_field(y, op->tmp, C(op->scale_re, -op->scale_im), V);
+        break;
+    }
+}
+
+typedef struct {
+    int converged;
+    int iterations;
+    double residual_norm;
+    double relative_residual;
+    int reason;
+} orc_cg_result;
+
+static double rel_res(double rn, double bn) {
+    if (bn == 0.0) return rn == 0.0 ? 0.0 : INFINITY;
+    return rn / bn;
+}
+static void set_result(orc_cg_result* r, int conv, int it, double rn, double bn, int reason) {
+    r->converged = conv;
+    r->iterations = it;
+    r->residual_norm = rn;
+    r->relative_residual = rel_res(rn, bn);
+    r->reason = reason;
+}
+
+/* cg!, src/LinearAlgebras/cg.jl:73-137, in its operation order. */
+int orc_cg(double* x, const double* b, double* r, double* p, double* Ap, const orc_op* op, long V, double rtol,
+           double atol, int maxiter, int dot_order, orc_cg_result* res) {
+    double d2[2];
+    orc_dot(b, b, V, dot_order, d2);
+    double bb = d2[0];
+    if (!isfinite(bb) || bb < 0.0) {
+        set_result(res, 0, 0, NAN, NAN, ORC_BREAKDOWN);
+        return 0;
+    }
+    double bnorm = sqrt(bb);
+    double tol = fmax(atol, rtol * bnorm);
+    op_apply(op, Ap, x, V);
+    memcpy(r, b, sizeof(double) * 24 * V);  /* axpby!(1, b, 0, r) */
+    orc_axpby(-1.0, Ap, 1.0, r, V);         /* axpby!(-1, Ap, 1, r) */
+    memcpy(p, r, sizeof(double) * 24 * V);  /* axpby!(1, r, 0, p) */
+    orc_dot(r, r, V, dot_order, d2);
+    double rr = d2[0];
+    if (!isfinite(rr) || rr < 0.0) {
+        set_result(res, 0, 0, NAN, bnorm, ORC_BREAKDOWN);
+        return 0;
+    }
+    double rn = sqrt(rr);
+    if (rn <= tol) {
+        set_result(res, 1, 0, rn, bnorm, ORC_CONVERGED);
+        return 0;
+    }
+    for (int it = 1; it <= maxiter; ++it) {
+        op_apply(op, Ap, p, V);
+        orc_dot(p, Ap, V, dot_order, d2);
+        double pAp = d2[0];
+        if (!isfinite(pAp)) {
+            set_result(res, 0, it - 1, rn, bnorm, ORC_BREAKDOWN);
+            return 0;
+        } else if (pAp <= 0.0) {
+            set_result(res, 0, it - 1, rn, bnorm, ORC_NONPOSITIVE_CURVATURE);
+            return 0;
+        }
+        double alpha = rr / pAp;
+        orc_axpby(alpha, p, 1.0, x, V);
+        orc_axpby(-alpha, Ap, 1.0, r, V);
+        orc_dot(r, r, V, dot_order, d2);
+        double rr_new = d2[0];
+        if (!isfinite(rr_new) || rr_new < 0.0) {
+            set_result(res, 0, it, NAN, bnorm, ORC_BREAKDOWN);
+            return 0;
+        }
+        rn = sqrt(rr_new);
+        if (rn <= tol) {
+            set_result(res, 1, it, rn, bnorm, ORC_CONVERGED);
+            return 0;
+        }
+        double beta = rr_new / rr;
+        orc_axpby(1.0, r, beta, p, V);
+        rr = rr_new;
+    }
+    set_result(res, 0, maxiter, rn, bnorm, ORC_MAXITER);
+    return 0;
+}
+
+/* Apply the normal operator once (y = A x), for true-residual checks. */
+void orc_op_apply(const orc_op* op, double* y, const double* x, long V) { op_apply(op, y, x, V); }
+
+/* ------------------------------------------------------------------------ */
+/* Force: link pullback and traceless anti-Hermitian projection             */
+/* ------------------------------------------------------------------------ */
+
+/* _wilson_halfspin_link_pullback_direction3! / _row3!,
+ * src/Operators/WilsonDiracOperator.jl:354-410, driven per site by
+ * _kernel_wilson_link_pullback! (ext/AD/WilsonDiracOperator.jl:62-91).
+ * dU[mu] += coefficient * value; dresult and psi are spinors (their x+mu
+ * values carry the spinor boundary phase, as after set_halo!). */
+void orc_wilson_pullback(double* dU, const double* dres, const double* psi, const int* L, const double* phases,
+                         double coefficient) {
+    geom g;
+    geom_init(&g, L, phases);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        int x[4];
+        site_coords(&g, s, x);
+        cx d0[12], p0[12];
+        load_spinor(dres, s, d0);
+        load_spinor(psi, s, p0);
+        for (int mu = 0; mu < 4; ++mu) {
+            int wrap;
+            long sp = nbr(&g, s, x, mu, +1, &wrap);
+            cx d1[12], p1[12];
+            load_spinor(dres, sp, d1);
+            load_spinor(psi, sp, p1);
+            if (wrap) {
+                scale_spinor(d1, g.ph[mu]);
+                scale_spinor(p1, g.ph[mu]);
+            }
+            cx plus_psi[6], minus_dres[6];
+            for (int c = 0; c < 3; ++c) {
+                half_project(p1, c, -1, mu, &plus_psi[2 * c], &plus_psi[2 * c + 1]);
+                half_project(d1, c, +1, mu, &minus_dres[2 * c], &minus_dres[2 * c + 1]);
+            }
+            for (int row = 0; row < 3; ++row) {
+                cx dplus1, dplus2, mpsi1, mpsi2;
+                half_project(d0, row, -1, mu, &dplus1, &dplus2);
+                half_project(p0, row, +1, mu, &mpsi1, &mpsi2);
+                for (int col = 0; col < 3; ++col) {
+                    cx v = cmul(dplus1, cconj(plus_psi[2 * col]));
+                    v = cadd(v, cmul(dplus2, cconj(plus_psi[2 * col + 1])));
+                    v = cadd(v, cmul(mpsi1, cconj(minus_dres[2 * col])));
+                    v = cadd(v, cmul(mpsi2, cconj(minus_dres[2 * col + 1])));
+                    double* o = dU + (((long)mu * g.V + s) * 9 + row * 3 + col) * 2;
+                    cx nv = cadd(C(o[0], o[1]), rmul(coefficient, v));
+                    o[0] = nv.re;
+                    o[1] = nv.im;
+                }
+            }
+        }
+    }
+}
+
+/* M = U G^dagger per link (row i of U times column j of G^dagger, as one
+ * muladd chain), then kernel_traceless_antihermitian! NC=3 (src/TA/TA.jl:68-116). */
+void orc_ta_force(double* Fout, const double* U, const double* G, long nlinks) {
+#pragma omp parallel for schedule(static)
+    for (long l = 0; l < nlinks; ++l) {
+        cx u[9], gg[9], m[9];
+        for (int k = 0; k < 9; ++k) {
+            u[k] = C(U[l * 18 + 2 * k], U[l * 18 + 2 * k + 1]);
+            gg[k] = C(G[l * 18 + 2 * k], G[l * 18 + 2 * k + 1]);
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                m[3 * i + j] = muladdmulti(u[3 * i], cconj(gg[3 * j]), u[3 * i + 1], cconj(gg[3 * j + 1]), u[3 * i + 2],
+                                           cconj(gg[3 * j + 2]));
+        const double fac13 = 1.0 / 3.0;
+        double tri = fac13 * ((m[0].im + m[4].im) + m[8].im);
+        cx y11 = C(0.0, m[0].im - tri), y22 = C(0.0, m[4].im - tri), y33 = C(0.0, m[8].im - tri);
+        cx x12 = csub(m[1], cconj(m[3]));
+        cx x13 = csub(m[2], cconj(m[6]));
+        cx x23 = csub(m[5], cconj(m[7]));
+        cx x21 = cneg(cconj(x12)), x31 = cneg(cconj(x13)), x32 = cneg(cconj(x23));
+        cx o[9] = {y11, rmul(0.5, x12), rmul(0.5, x13), rmul(0.5, x21), y22, rmul(0.5, x23),
+                   rmul(0.5, x31), rmul(0.5, x32), y33};
+        for (int k = 0; k < 9; ++k) {
+            Fout[l * 18 + 2 * k] = o[k].re;
+            Fout[l * 18 + 2 * k + 1] = o[k].im;
+        }
+    }
+}
+
+/* dSFdU for the Wilson operator (ext/LatticeMatricesEnzymeExt.jl:387-460):
+ * eta = (D^dagger D)^-1 phi (zero start), chi = D eta, and the Enzyme reverse
+ * rule of mul!(., D, eta) seeded with dresult = -2 chi (the pullback of
+ * g = -2 Re dot(chi, D eta), ext/AD/AD.jl:4014-4032) with coefficient -kappa.
+ * G is overwritten; eta/chi are outputs; work needs 4 spinor fields. */
+int orc_dsfdu(double* G, double* eta, double* chi, const double* U, const double* phi, const int* L,
+              const double* phases, double kappa, double rtol, int maxiter, int dot_order, double* work,
+              orc_cg_result* res) {
+    geom g;
+    geom_init(&g, L, phases);
+    long V = g.V;
+    double *r = work, *p = work + 24 * V, *Ap = work + 48 * V, *tmp = work + 72 * V;
+    orc_op op;
+    memset(&op, 0, sizeof(op));
+    op.kind = ORC_OP_DDAGD_WILSON;
+    op.U = U;
+    memcpy(op.L, L, sizeof(op.L));
+    for (int k = 0; k < 8; ++k) op.phases[k] = phases ? phases[k] : (k % 2 ? 0.0 : 1.0);
+    op.kappa = kappa;
+    op.tmp = tmp;
+    memset(eta, 0, sizeof(double) * 24 * V);
+    orc_cg(eta, phi, r, p, Ap, &op, V, rtol, 0.0, maxiter, dot_order, res);
+    orc_wilson(chi, eta, U, L, phases, kappa, 0, 0);
+    double* dres = tmp;
+    for (long i = 0; i < 24 * V; ++i) dres[i] = -2.0 * chi[i];
+    memset(G, 0, sizeof(double) * 4 * V * 18);
+    orc_wilson_pullback(G, dres, eta, L, phases, -kappa);
+    return 0;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

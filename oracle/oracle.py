@@ -1,0 +1,240 @@
+"""ctypes front end of the CPU oracle (liblmx_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker (or as the timed CPU
+restatement for cpu_baseline).  The product package never imports this module.
+
+Array conventions (numpy, C order, complex128):
+  spinor  psi[V, 4, 3]        (site, spin, colour)  == the reference's
+                              (3,4,X,Y,Z,T) column-major array without halo
+  links   U[4, V, 3, 3]       (mu, site, row, col)
+  clover  F[6, V, 3, 3]       planes (12,13,14,23,24,34)
+Sites are x-fastest: site = x + Lx*(y + Ly*(z + Lz*t)).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblmx_oracle.so")
+
+CONVERGED, MAXITER, NONPOSITIVE_CURVATURE, BREAKDOWN = 0, 1, 2, 3
+REASONS = {CONVERGED: "converged", MAXITER: "maximum_iterations",
+           NONPOSITIVE_CURVATURE: "nonpositive_curvature", BREAKDOWN: "numerical_breakdown"}
+
+OP_DDAGD_WILSON, OP_DDAGD_CLOVER, OP_SCALE, OP_DDAGD_SCALE = 0, 1, 2, 3
+
+
+class _OrcOp(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("U", ctypes.c_void_p), ("F", ctypes.c_void_p),
+                ("L", ctypes.c_int * 4), ("phases", ctypes.c_double * 8),
+                ("kappa", ctypes.c_double), ("csw", ctypes.c_double),
+                ("scale_re", ctypes.c_double), ("scale_im", ctypes.c_double),
+                ("tmp", ctypes.c_void_p)]
+
+
+class _OrcResult(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("residual_norm", ctypes.c_double), ("relative_residual", ctypes.c_double),
+                ("reason", ctypes.c_int)]
+
+
+@dataclass
+class CGResult:
+    converged: bool
+    iterations: int
+    residual_norm: float
+    relative_residual: float
+    reason: str
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {_LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        L.orc_wilson.argtypes = [P, P, P, P, P, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_dot.argtypes = [P, P, ctypes.c_long, ctypes.c_int, P]
+        L.orc_axpby.argtypes = [ctypes.c_double, P, ctypes.c_double, P, ctypes.c_long]
+        L.orc_clover_field_strength.argtypes = [P, P, P]
+        L.orc_wilson_clover.argtypes = [P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_cg.argtypes = [P, P, P, P, P, ctypes.POINTER(_OrcOp), ctypes.c_long, ctypes.c_double,
+                             ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_OrcResult)]
+        L.orc_op_apply.argtypes = [ctypes.POINTER(_OrcOp), P, P, ctypes.c_long]
+        L.orc_wilson_pullback.argtypes = [P, P, P, P, P, ctypes.c_double]
+        L.orc_ta_force.argtypes = [P, P, P, ctypes.c_long]
+        L.orc_dsfdu.argtypes = [P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_int, P, ctypes.POINTER(_OrcResult)]
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _c(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dims(L):
+    return (ctypes.c_int * 4)(*[int(v) for v in L])
+
+
+def _phases(phases):
+    ph = np.zeros(8, dtype=np.float64)
+    for d in range(4):
+        z = complex(phases[d]) if phases is not None else 1.0 + 0j
+        ph[2 * d], ph[2 * d + 1] = z.real, z.imag
+    return ph
+
+
+def volume(L):
+    return int(np.prod([int(v) for v in L]))
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+def wilson(U, psi, L, kappa, phases=(1, 1, 1, -1), dagger=False, donly=False):
+    """D_W psi (dagger: D_W^dagger psi; donly: hopping-only, coefficient 1/2)."""
+    V = volume(L)
+    U = _c(U, (4, V, 3, 3))
+    psi = _c(psi, (V, 4, 3))
+    out = np.empty_like(psi)
+    ph = _phases(phases)
+    lib().orc_wilson(_ptr(out), _ptr(psi), _ptr(U), _dims(L), _ptr(ph), float(kappa), int(dagger), int(donly))
+    return out
+
+
+def dot(a, b, order=1):
+    a = _c(a)
+    b = _c(b)
+    V = a.size // 12
+    out = np.zeros(2, dtype=np.float64)
+    lib().orc_dot(_ptr(a), _ptr(b), V, int(order), _ptr(out))
+    return complex(out[0], out[1])
+
+
+def clover_field_strength(U, L):
+    V = volume(L)
+    U = _c(U, (4, V, 3, 3))
+    F = np.empty((6, V, 3, 3), dtype=np.complex128)
+    lib().orc_clover_field_strength(_ptr(F), _ptr(U), _dims(L))
+    return F
+
+
+def wilson_clover(U, F, psi, L, kappa, csw, phases=(1, 1, 1, -1), dagger=False):
+    V = volume(L)
+    U = _c(U, (4, V, 3, 3))
+    F = _c(F, (6, V, 3, 3))
+    psi = _c(psi, (V, 4, 3))
+    out = np.empty_like(psi)
+    ph = _phases(phases)
+    lib().orc_wilson_clover(_ptr(out), _ptr(psi), _ptr(U), _ptr(F), _dims(L), _ptr(ph), float(kappa),
+                            float(csw), int(dagger))
+    return out
+
+
+class NormalOp:
+    """Operator description for the oracle CG: D^dagger D of Wilson/clover, a
+    complex scale (mul!(y, s, x)) or D^dagger D of a scale."""
+
+    def __init__(self, kind, L, U=None, F=None, kappa=0.0, csw=0.0, phases=(1, 1, 1, -1), scale=0j):
+        self.V = volume(L)
+        self._keep = []
+        self.s = _OrcOp()
+        self.s.kind = kind
+        if U is not None:
+            U = _c(U, (4, self.V, 3, 3))
+            self._keep.append(U)
+            self.s.U = U.ctypes.data
+        if F is not None:
+            F = _c(F, (6, self.V, 3, 3))
+            self._keep.append(F)
+            self.s.F = F.ctypes.data
+        for d in range(4):
+            self.s.L[d] = int(L[d])
+        ph = _phases(phases)
+        for k in range(8):
+            self.s.phases[k] = ph[k]
+        self.s.kappa, self.s.csw = float(kappa), float(csw)
+        self.s.scale_re, self.s.scale_im = complex(scale).real, complex(scale).imag
+        tmp = np.zeros((self.V, 4, 3), dtype=np.complex128)
+        self._keep.append(tmp)
+        self.s.tmp = tmp.ctypes.data
+
+    def apply(self, x):
+        x = _c(x, (self.V, 4, 3))
+        y = np.empty_like(x)
+        lib().orc_op_apply(ctypes.byref(self.s), _ptr(y), _ptr(x), self.V)
+        return y
+
+
+def cg(op: NormalOp, b, x0=None, rtol=1e-10, atol=0.0, maxiter=5000, order=1):
+    """cg!(x, A, b, r, p, Ap; rtol, atol, maxiter); returns (x, CGResult)."""
+    V = op.V
+    b = _c(b, (V, 4, 3))
+    x = np.zeros_like(b) if x0 is None else _c(x0, (V, 4, 3)).copy()
+    r = np.zeros_like(b)
+    p = np.zeros_like(b)
+    Ap = np.zeros_like(b)
+    res = _OrcResult()
+    lib().orc_cg(_ptr(x), _ptr(b), _ptr(r), _ptr(p), _ptr(Ap), ctypes.byref(op.s), V, float(rtol), float(atol),
+                 int(maxiter), int(order), ctypes.byref(res))
+    return x, CGResult(bool(res.converged), int(res.iterations), float(res.residual_norm),
+                       float(res.relative_residual), REASONS[int(res.reason)])
+
+
+def wilson_pullback(dres, psi, L, coefficient, phases=(1, 1, 1, -1), dU=None):
+    V = volume(L)
+    dres = _c(dres, (V, 4, 3))
+    psi = _c(psi, (V, 4, 3))
+    dU = np.zeros((4, V, 3, 3), dtype=np.complex128) if dU is None else _c(dU, (4, V, 3, 3)).copy()
+    ph = _phases(phases)
+    lib().orc_wilson_pullback(_ptr(dU), _ptr(dres), _ptr(psi), _dims(L), _ptr(ph), float(coefficient))
+    return dU
+
+
+def ta_force(U, G):
+    U = _c(U)
+    G = _c(G, U.shape)
+    F = np.empty_like(U)
+    lib().orc_ta_force(_ptr(F), _ptr(U), _ptr(G), U.size // 9)
+    return F
+
+
+def dsfdu(U, phi, L, kappa, phases=(1, 1, 1, -1), rtol=1e-12, maxiter=20000, order=1):
+    """Returns (G, eta, chi, CGResult) with G = dS/dU in the Enzyme convention."""
+    V = volume(L)
+    U = _c(U, (4, V, 3, 3))
+    phi = _c(phi, (V, 4, 3))
+    G = np.empty((4, V, 3, 3), dtype=np.complex128)
+    eta = np.empty((V, 4, 3), dtype=np.complex128)
+    chi = np.empty_like(eta)
+    work = np.empty((4, V, 4, 3), dtype=np.complex128)
+    ph = _phases(phases)
+    res = _OrcResult()
+    lib().orc_dsfdu(_ptr(G), _ptr(eta), _ptr(chi), _ptr(U), _ptr(phi), _dims(L), _ptr(ph), float(kappa),
+                    float(rtol), int(maxiter), int(order), _ptr(work), ctypes.byref(res))
+    return G, eta, chi, CGResult(bool(res.converged), int(res.iterations), float(res.residual_norm),
+                                 float(res.relative_residual), REASONS[int(res.reason)])

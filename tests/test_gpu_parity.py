@@ -1,0 +1,218 @@
+"""GPU parity of the HIP library (through the C ABI) against the CPU oracle.
+
+FP64 bar: relative error <= 1e-13 (the FP64 tolerance of the path); the FP64
+stencil, clover, BLAS-1 and CG kernels evaluate the reference's operations in
+the reference's order, so they are also checked for bit equality."""
+import numpy as np
+import pytest
+
+import dense_reference as dr
+import oracle as orc
+import lmx
+from lmx import generators as gen
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def rand_c(rng, shape):
+    return rng.random(shape) + 1j * rng.random(shape)
+
+
+CASES = [
+    ((2, 2, 2, 2), "rand", (1, 1, 1, -1)),
+    ((4, 2, 3, 2), "rand", (1, 1, 1, -1)),
+    ((4, 4, 4, 4), "haar", (1, 1, 1, -1)),
+    ((3, 2, 2, 4), "rand", (np.exp(0.3j), 1, -1j, -1)),
+    ((8, 4, 6, 2), "haar", (-1, 1, 1, -1)),
+]
+
+
+def make_links(L, kind, seed):
+    V = int(np.prod(L))
+    if kind == "rand":
+        return rand_c(np.random.default_rng(seed), (4, V, 3, 3))
+    return gen.links(L, kind, seed)
+
+
+@pytest.mark.parametrize("L,kind,phases", CASES)
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_wilson_apply_matches_oracle(gpu, L, kind, phases, flags):
+    print(f"wilson parity L={L} kind={kind} flags={flags}", flush=True)
+    U = make_links(L, kind, 1)
+    psi = rand_c(np.random.default_rng(2), (int(np.prod(L)), 4, 3))
+    lat = lmx.Lattice(L, phases)
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracOperator4D(Ug, 0.117)
+    A = lmx.WilsonDiracOperator4D_Donly(Ug) if flags & 2 else D
+    if flags & 1:
+        A = lmx.adjoint(A)
+    x = lat.spinor_from_numpy(psi)
+    y = lat.spinor()
+    lmx.mul_(y, A, x)
+    got = y.to_numpy()
+    ref = orc.wilson(U, psi, L, 0.117, phases, dagger=bool(flags & 1), donly=bool(flags & 2))
+    assert rel(got, ref) <= TOL
+    assert np.array_equal(got, ref), f"not bit-exact: max rel {rel(got, ref):.3e}"
+
+
+def test_layout_roundtrip_with_halo(gpu):
+    L = (4, 3, 2, 4)
+    phases = (1, 1j, 1, -1)
+    V = int(np.prod(L))
+    psi = gen.gaussian_spinor(L, 3)
+    lat = lmx.Lattice(L, phases)
+    x = lat.spinor_from_numpy(psi)
+    pad = x.to_padded(1).reshape(L[3] + 2, L[2] + 2, L[1] + 2, L[0] + 2, 4, 3)
+    core = dr.grid(L, psi)
+    # expected padded array: periodic extension with phase^wraps per direction
+    idx = [np.arange(-1, L[d] + 1) for d in range(4)]
+    exp = np.empty_like(pad)
+    for it, t in enumerate(idx[3]):
+        for iz, z in enumerate(idx[2]):
+            for iy, y in enumerate(idx[1]):
+                for ix, xx in enumerate(idx[0]):
+                    c = (xx % L[0], y % L[1], z % L[2], t % L[3])
+                    w = [(v - (v % L[d])) // L[d] for d, v in enumerate((xx, y, z, t))]
+                    f = np.prod([complex(phases[d]) ** w[d] for d in range(4)])
+                    exp[it, iz, iy, ix] = core[c[3], c[2], c[1], c[0]] * f
+    assert np.array_equal(pad, exp)
+    # import from the padded array ignores the halo
+    y = lat.spinor_from_numpy(pad, nw=1)
+    assert np.array_equal(y.to_numpy(), psi)
+    # links round trip
+    U = gen.links(L, "haar", 4)
+    assert np.array_equal(lat.gauge_from_numpy(U).to_numpy(), U)
+
+
+def test_blas_matches_oracle(gpu):
+    L = (8, 4, 4, 6)
+    a = gen.gaussian_spinor(L, 5)
+    b = gen.gaussian_spinor(L, 6)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    A, B = lat.spinor_from_numpy(a), lat.spinor_from_numpy(b)
+    d = lmx.dot(A, B)
+    ref = orc.dot(a, b, order=1)
+    assert d == ref
+    lmx.axpby_(0.37, A, -1.25, B)
+    assert np.array_equal(B.to_numpy(), 0.37 * a + (-1.25) * b)
+
+
+def test_clover_matches_oracle(gpu):
+    L = (4, 4, 2, 4)
+    U = gen.links(L, "weak", 7, eps=0.15)
+    psi = gen.gaussian_spinor(L, 8)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracCloverOperator4D(Ug, 0.1235, 1.0)
+    F = orc.clover_field_strength(U, L)
+    Fg = D.clover_planes()
+    assert rel(Fg, F) <= TOL
+    assert np.array_equal(Fg, F)
+    x = lat.spinor_from_numpy(psi)
+    y = lat.spinor()
+    for dag in (False, True):
+        lmx.mul_(y, lmx.adjoint(D) if dag else D, x)
+        ref = orc.wilson_clover(U, F, psi, L, 0.1235, 1.0, dagger=dag)
+        got = y.to_numpy()
+        assert rel(got, ref) <= TOL
+        assert np.array_equal(got, ref)
+
+
+def test_cg_config1_matches_oracle(gpu):
+    # 4^4 Haar, kappa 0.125, (1,1,1,-1), rtol 1e-10 from zero
+    L = (4, 4, 4, 4)
+    U = gen.links(L, "haar", 1234)
+    b = gen.gaussian_spinor(L, 1235)
+    op = orc.NormalOp(orc.OP_DDAGD_WILSON, L, U=U, kappa=0.125)
+    xo, ro = orc.cg(op, b, rtol=1e-10, order=1)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    D = lmx.WilsonDiracOperator4D(lat.gauge_from_numpy(U), 0.125)
+    bg = lat.spinor_from_numpy(b)
+    x, r, p, Ap, tmp = (lat.spinor() for _ in range(5))
+    res = lmx.cg_(x, lmx.DdagDOp(D, tmp), bg, r, p, Ap, rtol=1e-10)
+    assert res.converged and ro.converged
+    assert res.iterations == ro.iterations
+    xg = x.to_numpy()
+    assert rel(xg, xo) <= TOL
+    assert np.array_equal(xg, xo)
+    assert res.residual_norm == ro.residual_norm
+    true_res = np.linalg.norm(op.apply(xg) - b) / np.linalg.norm(b)
+    assert true_res <= 1e-10 * 1.01
+
+
+def test_cg_known_answers_generic(gpu):
+    # test/cg.jl:29-64,76-95 through the generic-operator CG
+    L = (1, 1, 1, 4)
+    b = (np.arange(1, 49, dtype=np.float64) + 0j).reshape(4, 4, 3)
+    lat = lmx.Lattice(L)
+    bg = lat.spinor_from_numpy(b)
+    x, r, p, Ap = (lat.spinor() for _ in range(4))
+    A = lmx.ScaledOperator(4.0)
+    res = lmx.cg_(x, A, bg, r, p, Ap, rtol=1e-13, maxiter=10)
+    assert res.converged and res.iterations == 1 and res.reason == "converged"
+    assert rel(x.to_numpy(), b / 4) <= 2e-13
+    lmx.clear_matrix_(x)
+    zb = lat.spinor()
+    res = lmx.cg_(x, A, zb, r, p, Ap, rtol=1e-13, maxiter=10)
+    assert res.converged and res.iterations == 0 and res.residual_norm == 0 and res.relative_residual == 0
+    lmx.clear_matrix_(x)
+    res = lmx.cg_(x, A, bg, r, p, Ap, maxiter=0)
+    assert not res.converged and res.reason == "maximum_iterations"
+    lmx.clear_matrix_(x)
+    res = lmx.cg_(x, lmx.ScaledOperator(-1.0), bg, r, p, Ap, maxiter=10)
+    assert not res.converged and res.reason == "nonpositive_curvature"
+    with pytest.raises(lmx.ArgumentError):
+        lmx.cg_(x, A, bg, x, p, Ap)
+    with pytest.raises(lmx.ArgumentError):
+        lmx.cg_(x, A, bg, r, p, Ap, rtol=-1)
+    with pytest.raises(lmx.ArgumentError):
+        lmx.cg_(x, A, bg, r, p, Ap, maxiter=-1)
+    # DdagDOp((2+i) I) = 5 I
+    tmp, out = lat.spinor(), lat.spinor()
+    N = lmx.DdagDOp(lmx.ScaledOperator(2 + 1j), tmp)
+    lmx.mul_(out, N, bg)
+    assert rel(out.to_numpy(), 5 * b) <= 1e-15
+    lmx.clear_matrix_(x)
+    res = lmx.solve_(x, N, bg, r, p, Ap, rtol=1e-13, maxiter=10)
+    assert res.converged and rel(x.to_numpy(), b / 5) <= 2e-13
+    with pytest.raises(lmx.ArgumentError):
+        lmx.mul_(out, lmx.DdagDOp(lmx.ScaledOperator(2 + 1j), bg), bg)
+
+
+def test_force_matches_oracle(gpu):
+    L = (4, 4, 4, 4)
+    U = gen.links(L, "haar", 51)
+    eta = gen.gaussian_spinor(L, 52)
+    phi = orc.wilson(U, eta, L, 0.125, dagger=True)
+    Go, Xo, chio, ro = orc.dsfdu(U, phi, L, 0.125, rtol=1e-10, order=1)
+    Fo = orc.ta_force(U, Go)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracOperator4D(Ug, 0.125)
+    G, F = lmx.GaugeField(lat), lmx.GaugeField(lat)
+    res, eta_g, chi_g = lmx.dSFdU(G, D, lat.spinor_from_numpy(phi), force=F, rtol=1e-10)
+    assert res.converged and res.iterations == ro.iterations
+    assert rel(eta_g.to_numpy(), Xo) <= TOL
+    assert rel(G.to_numpy(), Go) <= TOL
+    assert rel(F.to_numpy(), Fo) <= TOL
+    assert np.array_equal(G.to_numpy(), Go)
+
+
+def test_pullback_matches_oracle(gpu):
+    L = (4, 2, 4, 4)
+    U = gen.links(L, "haar", 71)
+    psi = gen.gaussian_spinor(L, 72)
+    w = gen.gaussian_spinor(L, 73)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracOperator4D(Ug, 0.125)
+    dU = lmx.GaugeField(lat)
+    lmx.wilson_pullback_(dU, D, lat.spinor_from_numpy(w), lat.spinor_from_numpy(psi), -0.125)
+    ref = orc.wilson_pullback(w, psi, L, -0.125)
+    assert np.array_equal(dU.to_numpy(), ref)
