@@ -1,12 +1,22 @@
 """Benchmark of the Wilson D^dagger D CG path on one MI355X.
 
 Workload (N=1): 32^4 Wilson D^dagger D, weak-field SU(3) links
-U = exp(i 0.2 H) (seeded), Gaussian source, kappa 0.124, antiperiodic in t,
-FP64 true relative residual 1e-10, zero start.  One step = one full solve.
-Inputs are generated on the host (numpy, seeded) and uploaded before the
-timed region; the timed region holds only device work.
+U = exp(i 0.2 H) (seeded), Gaussian sources, kappa 0.124, antiperiodic in t,
+FP64 true relative residual 1e-10, zero start.
+  - plain FP64 GPU CG (the reference algorithm cg!, one RHS), and
+  - the mixed-precision multi-RHS CG (FP16-stored links, FP32 spinors and
+    math, FP64 reliable updates / reductions / solution) on `--nrhs` sources.
+One step = one full solve of the mixed multi-RHS batch; `value` is its
+time-to-solution per RHS.  Inputs are generated on the host (numpy, seeded)
+and uploaded before the timed region; the timed region holds device work
+only (plus the solver's host convergence polls).
 
-Prints one JSON line (rank 0).  Run: python bench.py [--steps K] [--warmup W]
+Also measured: the FP64 D_W launch (roofline kernel, 960 B/site algorithmic:
+links 576 + psi 192 + out 192) timed with HIP events on the library stream,
+the mixed multi-RHS dslash (144/n + 192 B/site/RHS), and a plain 1 GiB copy
+kernel used to calibrate the rocprof byte counters.
+
+Prints one JSON line.  Run: python bench.py [--steps K] [--warmup W]
 """
 from __future__ import annotations
 
@@ -29,39 +39,53 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dims", type=str, default="32,32,32,32")
     ap.add_argument("--kappa", type=float, default=0.124)
     ap.add_argument("--eps", type=float, default=0.2)
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--nrhs", type=int, default=12)
+    ap.add_argument("--precision", type=str, default="f16", choices=["f16", "f32"])
+    ap.add_argument("--rhs-block", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--dslash-reps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=1)
     return ap.parse_args()
 
 
-def time_dslash(lat, D, x, y, reps, flags_dagger=False):
-    """Average duration of one D_W launch, measured with HIP events on the
-    stream the library launches on (the context stream = torch's current)."""
+def timed(lat, fn, reps):
+    """Average duration of fn() over reps launches, HIP events on the library stream."""
     import torch
-    import lmx
-    A = lmx.adjoint(D) if flags_dagger else D
     for _ in range(3):
-        lmx.mul_(y, A, x)
+        fn()
     s = torch.cuda.Event(enable_timing=True)
     e = torch.cuda.Event(enable_timing=True)
     s.record(lat.stream)
     for _ in range(reps):
-        lmx.mul_(y, A, x)
+        fn()
     e.record(lat.stream)
     e.synchronize()
     return s.elapsed_time(e) / reps
 
 
-def cpu_baseline(U, b, L, kappa, iters):
+def copy_calibration(lat):
+    """A plain device copy of 1 GiB (read 1 GiB + write 1 GiB): the reference
+    kernel for scaling the rocprof FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device=lat.device)
+    b = torch.empty_like(a)
+    ms = timed(lat, lambda: b.copy_(a), 10)
+    del a, b
+    return ms
+
+
+def cpu_baseline(U, b, L, kappa, iters, gpu_iterations):
     """Oracle (C/OpenMP restatement of the reference CPU algorithm) on the
-    same 32^4 inputs: a bounded sample of `iters` CG iterations."""
+    same inputs: a bounded sample of `iters` FP64 CG iterations."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     threads = orc.num_threads()
@@ -69,14 +93,17 @@ def cpu_baseline(U, b, L, kappa, iters):
     t0 = time.perf_counter()
     _, res = orc.cg(op, b, rtol=1e-30, maxiter=iters, order=1)
     dt = time.perf_counter() - t0
-    return dt, res.iterations, threads
+    per_iter = dt / max(res.iterations, 1)
+    return {"value": per_iter * gpu_iterations, "unit": "s/RHS", "cores": threads, "kind": "port",
+            "sample": f"{res.iterations} FP64 CG iterations of the same 32^4 solve on the C/OpenMP oracle "
+                      f"({threads} threads, {per_iter:.3f} s/iteration) x {gpu_iterations} iterations"}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design "
-                                   "note only)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the multi-GPU sharding of the "
+                                   "lattice is a design note only)"}))
         sys.exit(2)
     import numpy as np
     import torch
@@ -86,76 +113,109 @@ def main():
 
     L = tuple(int(v) for v in args.dims.split(","))
     V = int(np.prod(L))
-    log(f"bench: generating inputs {L} weak-field eps={args.eps}")
+    n = args.nrhs
+    log(f"bench: generating inputs {L} weak-field eps={args.eps}, {n} Gaussian sources")
     U = gen.links(L, "weak", 20240601, eps=args.eps)
-    b = gen.gaussian_spinor(L, 20240602)
+    b = gen.gaussian_spinor(L, 20240602, nrhs=n)
     lat = lmx.Lattice(L, (1, 1, 1, -1))
     Ug = lat.gauge_from_numpy(U)
     D = lmx.WilsonDiracOperator4D(Ug, args.kappa)
     bg = lat.spinor_from_numpy(b)
-    x, r, p, Ap, tmp = (lat.spinor() for _ in range(5))
+    b0 = bg.rhs(0)
+    x1, r, p, Ap, tmp = (lat.spinor() for _ in range(5))
     N = lmx.DdagDOp(D, tmp)
 
-    # ---- dslash roofline (FP64 D_W, 960 B/site algorithmic) -------------
-    t_ms = time_dslash(lat, D, bg, tmp, args.dslash_reps)
-    bytes_per_launch = 960.0 * V
-    achieved = bytes_per_launch / (t_ms * 1e-3) / 1e9
-    log(f"bench: FP64 D_W {t_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
+    # ---- dslash roofline (FP64 D_W) and calibration copy ------------------
+    t_d64 = timed(lat, lambda: lmx.mul_(tmp, D, b0), args.dslash_reps)
+    bytes_d64 = 960.0 * V
+    ach_d64 = bytes_d64 / (t_d64 * 1e-3) / 1e9
+    log(f"bench: FP64 D_W {t_d64 * 1e3:.1f} us/launch, {ach_d64:.0f} GB/s algorithmic")
+    t_copy = copy_calibration(lat)
+    log(f"bench: 1 GiB copy {t_copy * 1e3:.1f} us ({2 * (1 << 30) / (t_copy * 1e-3) / 1e9:.0f} GB/s)")
 
-    # ---- FP64 CG time-to-solution ---------------------------------------
-    def solve():
-        lmx.clear_matrix_(x)
-        return lmx.cg_(x, N, bg, r, p, Ap, rtol=args.rtol, maxiter=20000)
+    # ---- plain FP64 CG (the reference algorithm), RHS 0 -------------------
+    def solve64():
+        lmx.clear_matrix_(x1)
+        return lmx.cg_(x1, N, b0, r, p, Ap, rtol=args.rtol, maxiter=20000)
+
+    res64 = solve64()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        res64 = solve64()
+    torch.cuda.synchronize()
+    t64 = (time.perf_counter() - t0) / args.fp64_steps
+    log(f"bench: FP64 CG {t64:.3f} s/RHS, {res64.iterations} iterations, converged={res64.converged}")
+    x64 = x1.to_numpy()
+
+    # ---- mixed multi-RHS CG ------------------------------------------------
+    tune = lmx.autotune(D, args.precision, n, os.path.join(ROOT, "gpurun_out", "lmx_tune.txt"))
+    bytes_dm = (144.0 if args.precision == "f16" else 288.0) / n + 192.0  # B/site/RHS, links shared by n RHS
+    ach_dm = bytes_dm * V / (tune["ms_per_apply"] * 1e-3) / 1e9
+    log(f"bench: autotuned {args.precision} dslash x{n}: rhs_block {tune['rhs_block'] >> 4}/{tune['rhs_block'] & 15} "
+        f"{tune['ms_per_apply'] * 1e3:.1f} us/RHS, {ach_dm:.0f} GB/s algorithmic")
+    xm = lat.spinor(n)
+
+    def solve_mixed():
+        lmx.clear_matrix_(xm)
+        return lmx.cg_batched_(xm, D, bg, precision=args.precision, rtol=args.rtol, use_graph=bool(args.graph),
+                               rhs_block=args.rhs_block)
 
     for w in range(args.warmup):
-        res = solve()
-        log(f"bench: warmup {w}: {res.iterations} iterations, converged={res.converged}")
+        resm = solve_mixed()
+        log(f"bench: warmup {w}: iterations {[q.iterations for q in resm]}")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        res = solve()
-        log(f"bench: step {k}: {res.iterations} iterations")
+        resm = solve_mixed()
+        log(f"bench: step {k}: iterations {[q.iterations for q in resm]} "
+            f"updates {[q.reliable_updates for q in resm]}")
     torch.cuda.synchronize()
-    t_solve = (time.perf_counter() - t0) / args.steps
-    # true residual in FP64
-    lmx.mul_(Ap, N, x)
-    lmx.axpby_(-1.0, bg, 1.0, Ap)
-    true_rel = float(np.sqrt(lmx.dot(Ap, Ap).real / lmx.dot(bg, bg).real))
-    log(f"bench: FP64 CG {t_solve:.3f} s/solve, {res.iterations} iterations, true rel residual {true_rel:.3e}")
+    tm = (time.perf_counter() - t0) / args.steps
+    per_rhs = tm / n
+    xm0 = xm.to_numpy()[0]
+    err0 = float(np.linalg.norm(xm0 - x64) / np.linalg.norm(x64))
+    true_res = [q.true_relative_residual for q in resm]
+    log(f"bench: mixed {args.precision} x{n}: {tm:.3f} s/batch = {per_rhs:.4f} s/RHS; speedup vs FP64 "
+        f"{t64 / per_rhs:.2f}x; max true rel residual {max(true_res):.3e}; RHS0 error vs FP64 {err0:.3e}")
 
     cpu = None
     if not args.no_cpu:
-        dt, it, threads = cpu_baseline(U, b, L, args.kappa, args.cpu_iters)
-        per_iter = dt / max(it, 1)
-        cpu = {"value": per_iter * res.iterations, "unit": "s/RHS", "cores": threads, "kind": "port",
-               "sample": f"{it} CG iterations of the 32^4 FP64 solve (oracle, {threads} OpenMP threads), "
-                         f"{per_iter:.3f} s/iteration x {res.iterations} iterations"}
-        log(f"bench: cpu baseline {per_iter:.3f} s/iteration on {threads} threads")
+        cpu = cpu_baseline(U, b[0], L, args.kappa, args.cpu_iters, res64.iterations)
+        log(f"bench: cpu baseline {cpu['sample']}")
 
     out = {
         "metric": "D^dagger D CG time-to-solution/RHS @1e-10 (mixed+multiRHS vs FP64); dslash HBM GB/s",
-        "value": t_solve,
+        "value": per_rhs,
         "unit": "s/RHS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t_solve * 1e3,
+        "ms_per_step": tm * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded weak-field SU(3) links eps=0.2, Gaussian source)",
-        "config": {"workload": f"Wilson D^dagger D CG {'x'.join(map(str, L))}, kappa {args.kappa}, "
-                               f"rtol {args.rtol}, FP64", "lattice": list(L), "kappa": args.kappa,
-                   "rhs": 1, "precision": "fp64"},
-        "iterations": res.iterations,
-        "true_relative_residual": true_rel,
-        "fp64_time_per_rhs_s": t_solve,
-        "dslash_gbs": achieved,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_wilson FP64 D_W", "bytes_per_launch": bytes_per_launch,
-                     "us_per_launch": t_ms * 1e3},
+        "dtype": f"{args.precision} links / f32 spinors+math, f64 reliable updates",
+        "data": "synthetic (seeded weak-field SU(3) links eps=0.2, Gaussian sources)",
+        "config": {"workload": f"Wilson D^dagger D CG {'x'.join(map(str, L))}, kappa {args.kappa}, rtol "
+                               f"{args.rtol}, {n} RHS mixed vs 1 RHS FP64",
+                   "lattice": list(L), "kappa": args.kappa, "rhs": n, "precision": args.precision},
+        "fp64_time_per_rhs_s": t64,
+        "fp64_iterations": res64.iterations,
+        "speedup_vs_fp64": t64 / per_rhs,
+        "mixed_iterations": [q.iterations for q in resm],
+        "mixed_reliable_updates": [q.reliable_updates for q in resm],
+        "mixed_true_relative_residual_max": max(true_res),
+        "mixed_rhs0_error_vs_fp64": err0,
+        "dslash_gbs": ach_d64,
+        "copy_1gib_us": t_copy * 1e3,
+        "mixed_dslash": {"rhs_block": tune["rhs_block"], "us_per_rhs": tune["ms_per_apply"] * 1e3,
+                         "algorithmic_bytes_per_site_per_rhs": bytes_dm, "achieved_gbs": ach_dm,
+                         "frac": ach_dm / HBM_PEAK_GBS},
+        "roofline": {"bound": "hbm", "achieved": ach_d64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach_d64 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_wilson FP64 D_W (32^4)", "bytes_per_launch": bytes_d64,
+                     "us_per_launch": t_d64 * 1e3},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

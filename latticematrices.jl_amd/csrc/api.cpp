@@ -503,3 +503,10 @@ int lmx_wilson_force(lmx_wilson* op, void* G, void* F, const void* phi, void* et
 }
 
 }  // extern "C"
+
+namespace lmx {
+int lmx_cg_ddagd_internal(lmx_wilson* op, double2* x, const double2* b, double2* r, double2* p, double2* Ap,
+                          double2* tmp, const lmx_cg_params* params, lmx_cg_result* result) {
+    return lmx_cg_ddagd(op, x, b, r, p, Ap, tmp, params, result);
+}
+}  // namespace lmx

@@ -80,7 +80,7 @@ struct lmx_wilson {
     __half2* clover16;
     bool lo_valid;
     // Autotuned launch shape per (precision, nrhs); 0 = untuned.
-    int tuned_block[3][13];
+    int tuned_block[3][65];
 };
 
 namespace lmx {
@@ -97,13 +97,14 @@ struct Epilogue {
     CGState* st;        // early-exit state (may be null)
     double* partials;
     unsigned* counter;
+    const int* gate = nullptr;  // if non-null, the launch does nothing while *gate == 0
 };
 
 int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
                       double2* out, const double2* in, int nrhs, int flags, const Epilogue& epi);
 
 int launch_wilson_lo(lmx_context* ctx, int link_prec, const void* U, const void* clover, double kappa, double csw,
-                     float2* out, const float2* in, int nrhs, int flags, int rhs_block, CGState* st);
+                     float2* out, const float2* in, int nrhs, int flags, int rhs_block, const int* gate);
 
 int launch_clover_build_f64(lmx_context* ctx, const double2* U, double2* clover);
 int launch_clover_export(lmx_context* ctx, const double2* clover, double2* F_out);

@@ -1,20 +1,616 @@
-// mixed.hip -- batched / mixed-precision D^dagger D CG (placeholder until the
-// mixed solver lands).
+// mixed.hip -- batched D^dagger D CG over many right-hand sides, plain FP64
+// or mixed precision with FP64 reliable updates.
+//
+// Mixed solver (per right-hand side i, all RHS advanced in lockstep by
+// batched kernels; every scalar lives on the device):
+//   inner CG in FP32 spinors with FP32 or FP16-stored links (FP32 math),
+//   p.Ap and |r|^2 accumulated in FP64, alpha/beta per RHS on the device;
+//   a reliable update replaces the inner residual by the FP64 true residual
+//   r = b - A x (A = D^dagger D in FP64) and folds the inner solution into the
+//   FP64 solution x, when |r| drops below delta * (largest |r| since the
+//   last update), after reliable_maxiter iterations, or when the inner
+//   residual reaches the tolerance.  Convergence is declared only on the
+//   FP64 true residual.  The update decision is a device flag that gates the
+//   FP64 kernels, so one iteration is a fixed sequence of launches that can be
+//   captured once in a hipGraph and replayed.
+// The reference solver this replaces is cg! (src/LinearAlgebras/cg.jl:73-137)
+// on DdagDOp (src/Operators/DiracOperators.jl:104-121).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "lmx_internal.h"
 
 namespace lmx {
 
-size_t cg_batched_workspace_impl(lmx_wilson*, int, const lmx_cg_params*) { return 0; }
+struct MixRhs {
+    double bnorm, tol, rr, rr_prev, pAp, alpha, beta, rref, rtrue;
+    int done, converged, reason, since, iterations, updates;
+};
+struct MixBatch {
+    int do_update;  // reliable update requested this iteration
+    int n_active;   // RHS still iterating
+    double delta;
+    int rmax;
+    int pad;
+};
 
-int cg_batched_impl(lmx_wilson*, double2*, const double2*, int, void*, size_t, const lmx_cg_params*,
-                    lmx_cg_result*) {
-    set_error("batched CG not built yet");
-    return LMX_ERR_UNSUPPORTED;
+struct MixPtrs {
+    float2* xlo;
+    float2* rlo;
+    float2* p;
+    float2* Ap;
+    float2* t32;
+    double2* r64;
+    double2* t64;
+    double* partials;
+    unsigned* counters;
+    MixRhs* rhs;
+    MixBatch* batch;
+};
+
+static inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+static size_t layout(lmx_context* ctx, int nrhs, MixPtrs* P, char* base) {
+    const size_t V = (size_t)ctx->V;
+    const size_t f32 = 12 * V * sizeof(float2) * nrhs, f64 = 12 * V * sizeof(double2) * nrhs;
+    const size_t nb = (size_t)nblocks(ctx->V);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    MixPtrs q;
+    q.xlo = (float2*)take(f32);
+    q.rlo = (float2*)take(f32);
+    q.p = (float2*)take(f32);
+    q.Ap = (float2*)take(f32);
+    q.t32 = (float2*)take(f32);
+    q.r64 = (double2*)take(f64);
+    q.t64 = (double2*)take(f64);
+    q.partials = (double*)take(2 * nb * nrhs * sizeof(double));
+    q.counters = (unsigned*)take(nrhs * sizeof(unsigned));
+    q.rhs = (MixRhs*)take(nrhs * sizeof(MixRhs));
+    q.batch = (MixBatch*)take(sizeof(MixBatch));
+    if (P) *P = q;
+    return off;
+}
+
+// ---------------------------------------------------------------------------
+// Reduction helper: every block of row i (blockIdx.y) reduces (a, b) over its
+// sites; the last block of the row calls fin(i, A, B) in thread 0.
+// ---------------------------------------------------------------------------
+template <class Fin>
+__device__ __forceinline__ void row_reduce(double a, double b, double* partials, unsigned* counters, Fin fin) {
+    __shared__ double lds4[4];
+    __shared__ int flag;
+    const int nb = gridDim.x, i = blockIdx.y, bx = blockIdx.x;
+    double ta = block_tree(a, lds4);
+    double tb = block_tree(b, lds4);
+    double* pa = partials + (size_t)(2 * i) * nb;
+    double* pb = partials + (size_t)(2 * i + 1) * nb;
+    if (threadIdx.x == 0) {
+        pa[bx] = ta;
+        pb[bx] = tb;
+        __threadfence();
+        unsigned prev = atomicAdd(counters + i, 1u);
+        flag = (prev == (unsigned)nb - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (flag) {
+        __threadfence();
+        double A = finish_partials(pa, nb, lds4);
+        double B = finish_partials(pb, nb, lds4);
+        if (threadIdx.x == 0) {
+            fin(i, A, B);
+            counters[i] = 0u;
+        }
+    }
+}
+
+__device__ __forceinline__ void retire(MixRhs* st, MixBatch* batch, int reason, int converged) {
+    st->done = 1;
+    st->reason = reason;
+    st->converged = converged;
+    atomicSub(&batch->n_active, 1);
+}
+
+// r64 = b - A x (A x already in r64); r_lo = r64; p = r_lo; x_lo = 0;
+// |r|^2 and |b|^2 per RHS -> initial state.
+__global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, double2* __restrict__ r64,
+                                                 float2* __restrict__ rlo, float2* __restrict__ p,
+                                                 float2* __restrict__ xlo, int V, double rtol, double atol,
+                                                 double* partials, unsigned* counters, MixRhs* rhs, MixBatch* batch) {
+    const int i = blockIdx.y;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    double rr = 0.0, bb = 0.0;
+    if (s < V) {
+        const long o = (long)i * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const long e = o + (long)k * V + s;
+            double2 bv = b[e], av = r64[e];
+            double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
+            r64[e] = rv;
+            float2 f = make_float2((float)rv.x, (float)rv.y);
+            rlo[e] = f;
+            p[e] = f;
+            xlo[e] = make_float2(0.f, 0.f);
+            rr += rv.x * rv.x + rv.y * rv.y;
+            bb += bv.x * bv.x + bv.y * bv.y;
+        }
+    }
+    row_reduce(rr, bb, partials, counters, [&](int i, double RR, double BB) {
+        MixRhs* st = rhs + i;
+        st->bnorm = sqrt(BB);
+        st->tol = fmax(atol, rtol * st->bnorm);
+        st->rr = RR;
+        st->rr_prev = RR;
+        st->rtrue = sqrt(RR);
+        st->rref = st->rtrue;
+        st->since = 0;
+        st->iterations = 0;
+        st->updates = 0;
+        st->beta = 0.0;
+        if (!isfinite(RR) || !isfinite(BB)) {
+            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+        } else if (st->rtrue <= st->tol) {
+            retire(st, batch, LMX_CG_CONVERGED, 1);
+        }
+    });
+}
+
+__global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta, int rmax) {
+    const int t = threadIdx.x;
+    if (t < nrhs) {
+        MixRhs z;
+        memset(&z, 0, sizeof(z));
+        z.reason = LMX_CG_MAXITER;
+        rhs[t] = z;
+    }
+    if (t == 0) {
+        batch->do_update = 0;
+        batch->n_active = nrhs;
+        batch->delta = delta;
+        batch->rmax = rmax;
+    }
+}
+
+// pAp_i = Re <p_i, Ap_i> (FP64 accumulation) -> alpha_i
+__global__ void __launch_bounds__(256) k_mx_pap(const float2* __restrict__ p, const float2* __restrict__ Ap, int V,
+                                                double* partials, unsigned* counters, MixRhs* rhs,
+                                                MixBatch* batch) {
+    const int i = blockIdx.y;
+    if (blockIdx.x == 0 && i == 0 && threadIdx.x == 0) batch->do_update = 0;
+    if (rhs[i].done) return;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (s < V) {
+        const long o = (long)i * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            float2 a = p[o + (long)k * V + s], c = Ap[o + (long)k * V + s];
+            v += (double)a.x * (double)c.x + (double)a.y * (double)c.y;
+        }
+    }
+    row_reduce(v, 0.0, partials, counters, [&](int i, double pAp, double) {
+        MixRhs* st = rhs + i;
+        st->pAp = pAp;
+        if (!isfinite(pAp)) {
+            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+        } else if (pAp <= 0.0) {
+            retire(st, batch, LMX_CG_NONPOSITIVE_CURVATURE, 0);
+        } else {
+            st->alpha = st->rr / pAp;
+        }
+    });
+}
+
+// x_lo += alpha p; r_lo -= alpha Ap; |r_lo|^2 -> beta, reliable-update trigger
+__global__ void __launch_bounds__(256) k_mx_update_xr(float2* __restrict__ xlo, float2* __restrict__ rlo,
+                                                      const float2* __restrict__ p, const float2* __restrict__ Ap,
+                                                      int V, double* partials, unsigned* counters, MixRhs* rhs,
+                                                      MixBatch* batch) {
+    const int i = blockIdx.y;
+    if (rhs[i].done) return;
+    const float a = (float)rhs[i].alpha;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (s < V) {
+        const long o = (long)i * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const long e = o + (long)k * V + s;
+            float2 pv = p[e], av = Ap[e], xv = xlo[e], rv = rlo[e];
+            xv.x = fmaf(a, pv.x, xv.x);
+            xv.y = fmaf(a, pv.y, xv.y);
+            rv.x = fmaf(-a, av.x, rv.x);
+            rv.y = fmaf(-a, av.y, rv.y);
+            xlo[e] = xv;
+            rlo[e] = rv;
+            v += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+        }
+    }
+    row_reduce(v, 0.0, partials, counters, [&](int i, double rr_new, double) {
+        MixRhs* st = rhs + i;
+        st->iterations += 1;
+        st->since += 1;
+        if (!isfinite(rr_new)) {
+            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+            return;
+        }
+        const double rn = sqrt(rr_new);
+        st->rref = fmax(st->rref, rn);
+        st->rr_prev = st->rr;
+        st->beta = rr_new / st->rr;
+        st->rr = rr_new;
+        const bool trigger = rn < batch->delta * st->rref || st->since >= batch->rmax || rn <= st->tol;
+        if (trigger) atomicOr(&batch->do_update, 1);
+    });
+}
+
+// reliable update, part 1: x += x_lo (FP64), x_lo = 0
+__global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float2* __restrict__ xlo, int V,
+                                                  const MixRhs* rhs, const MixBatch* batch, int force) {
+    const int i = blockIdx.y;
+    if ((!force && !batch->do_update) || rhs[i].done) return;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= V) return;
+    const long o = (long)i * 12 * V;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        const long e = o + (long)k * V + s;
+        float2 l = xlo[e];
+        double2 h = x[e];
+        x[e] = make_double2(h.x + (double)l.x, h.y + (double)l.y);
+        xlo[e] = make_float2(0.f, 0.f);
+    }
+}
+
+// reliable update, part 2: r64 = b - A x (A x in r64), r_lo = r64, true |r|
+__global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__ b, double2* __restrict__ r64,
+                                                     float2* __restrict__ rlo, int V, double* partials,
+                                                     unsigned* counters, MixRhs* rhs, MixBatch* batch, int force) {
+    const int i = blockIdx.y;
+    if ((!force && !batch->do_update) || rhs[i].done) return;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (s < V) {
+        const long o = (long)i * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const long e = o + (long)k * V + s;
+            double2 bv = b[e], av = r64[e];
+            double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
+            r64[e] = rv;
+            rlo[e] = make_float2((float)rv.x, (float)rv.y);
+            v += rv.x * rv.x + rv.y * rv.y;
+        }
+    }
+    row_reduce(v, 0.0, partials, counters, [&](int i, double rr, double) {
+        MixRhs* st = rhs + i;
+        st->rtrue = sqrt(rr);
+        st->updates += 1;
+        if (!isfinite(rr)) {
+            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+        } else if (st->rtrue <= st->tol) {
+            retire(st, batch, LMX_CG_CONVERGED, 1);
+        } else {
+            st->beta = rr / st->rr_prev;
+            st->rr = rr;
+            st->rref = st->rtrue;
+            st->since = 0;
+        }
+    });
+}
+
+// p = r_lo + beta p
+__global__ void __launch_bounds__(256) k_mx_update_p(const float2* __restrict__ rlo, float2* __restrict__ p, int V,
+                                                     const MixRhs* rhs) {
+    const int i = blockIdx.y;
+    if (rhs[i].done) return;
+    const float be = (float)rhs[i].beta;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= V) return;
+    const long o = (long)i * 12 * V;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        const long e = o + (long)k * V + s;
+        float2 r = rlo[e], q = p[e];
+        p[e] = make_float2(fmaf(be, q.x, r.x), fmaf(be, q.y, r.y));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host driver
+// ---------------------------------------------------------------------------
+struct MixRun {
+    lmx_wilson* op;
+    lmx_context* ctx;
+    MixPtrs w;
+    double2* x;
+    const double2* b;
+    int nrhs;
+    int link_prec;
+    int rhs_block;
+};
+
+static int apply_ddagd64(MixRun& R, double2* out, const double2* in, double2* tmp, const int* gate) {
+    Epilogue e{0, nullptr, nullptr, nullptr, nullptr, gate};
+    lmx_wilson* op = R.op;
+    LMX_TRY(launch_wilson_f64(R.ctx, op->links, op->clover64, op->kappa, op->csw, tmp, in, R.nrhs, 0, e));
+    return launch_wilson_f64(R.ctx, op->links, op->clover64, op->kappa, op->csw, out, tmp, R.nrhs, LMX_DAGGER, e);
+}
+
+static int apply_ddagd_lo(MixRun& R) {
+    lmx_wilson* op = R.op;
+    const void* U = R.link_prec == LMX_F16 ? (const void*)op->links16 : (const void*)op->links32;
+    const void* F = op->clover64 ? (R.link_prec == LMX_F16 ? (const void*)op->clover16 : (const void*)op->clover32)
+                                 : nullptr;
+    const int* gate = &R.w.batch->n_active;
+    LMX_TRY(launch_wilson_lo(R.ctx, R.link_prec, U, F, op->kappa, op->csw, R.w.t32, R.w.p, R.nrhs, 0, R.rhs_block,
+                             gate));
+    return launch_wilson_lo(R.ctx, R.link_prec, U, F, op->kappa, op->csw, R.w.Ap, R.w.t32, R.nrhs, LMX_DAGGER,
+                            R.rhs_block, gate);
+}
+
+static int enqueue_iteration(MixRun& R) {
+    lmx_context* c = R.ctx;
+    const int V = (int)c->V;
+    dim3 grid(nblocks(c->V), R.nrhs), blk(256);
+    LMX_TRY(apply_ddagd_lo(R));
+    hipLaunchKernelGGL(k_mx_pap, grid, blk, 0, c->stream, R.w.p, R.w.Ap, V, R.w.partials, R.w.counters, R.w.rhs,
+                       R.w.batch);
+    hipLaunchKernelGGL(k_mx_update_xr, grid, blk, 0, c->stream, R.w.xlo, R.w.rlo, R.w.p, R.w.Ap, V, R.w.partials,
+                       R.w.counters, R.w.rhs, R.w.batch);
+    hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, R.x, R.w.xlo, V, R.w.rhs, R.w.batch, 0);
+    LMX_TRY(apply_ddagd64(R, R.w.r64, R.x, R.w.t64, &R.w.batch->do_update));
+    hipLaunchKernelGGL(k_mx_true_res, grid, blk, 0, c->stream, R.b, R.w.r64, R.w.rlo, V, R.w.partials,
+                       R.w.counters, R.w.rhs, R.w.batch, 0);
+    hipLaunchKernelGGL(k_mx_update_p, grid, blk, 0, c->stream, R.w.rlo, R.w.p, V, R.w.rhs);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+size_t cg_batched_workspace_impl(lmx_wilson* op, int nrhs, const lmx_cg_params* params) {
+    if (params->precision == LMX_F64) return align256(4 * 12 * (size_t)op->ctx->V * sizeof(double2));
+    return layout(op->ctx, std::max(nrhs, 1), nullptr, nullptr);
+}
+
+int lmx_cg_ddagd_internal(lmx_wilson* op, double2* x, const double2* b, double2* r, double2* p, double2* Ap,
+                          double2* tmp, const lmx_cg_params* params, lmx_cg_result* result);
+
+static int default_rhs_block(lmx_wilson* op, int prec, int nrhs) {
+    const int t = (prec >= 0 && prec < 3 && nrhs >= 0 && nrhs <= 64) ? op->tuned_block[prec][nrhs] : 0;
+    if (t > 0) return t;
+    // all RHS in one launch, two per work item when that divides evenly
+    const int per = std::min(nrhs, 12);
+    const int nr = (per % 2 == 0) ? 2 : 1;
+    return (per << 4) | nr;
+}
+
+int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void* work, size_t bytes,
+                    const lmx_cg_params* params, lmx_cg_result* results) {
+    lmx_context* c = op->ctx;
+    const size_t need = cg_batched_workspace_impl(op, nrhs, params);
+    LMX_CHECK(work != nullptr || need == 0, LMX_ERR_ARG, "workspace is NULL");
+    LMX_CHECK(bytes >= need, LMX_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu required", bytes, need);
+    if (nrhs == 0) return LMX_OK;
+    const size_t fs = 12 * (size_t)c->V;
+    if (params->precision == LMX_F64) {
+        // plain FP64 cg! per right-hand side, the reference algorithm
+        double2* w = (double2*)work;
+        for (int i = 0; i < nrhs; ++i)
+            LMX_TRY(lmx_cg_ddagd_internal(op, x + i * fs, b + i * fs, w, w + fs, w + 2 * fs, w + 3 * fs, params,
+                                          results + i));
+        return LMX_OK;
+    }
+    LMX_CHECK(params->precision == LMX_F32 || params->precision == LMX_F16, LMX_ERR_ARG, "unknown precision %d",
+              params->precision);
+    LMX_CHECK(nrhs <= kMaxStates, LMX_ERR_ARG, "at most %d right-hand sides per batch", kMaxStates);
+    LMX_CHECK(params->reliable_delta > 0.0 && params->reliable_delta < 1.0, LMX_ERR_ARG,
+              "reliable_delta must be in (0, 1)");
+    LMX_TRY(ensure_lo_links(op));
+    MixRun R;
+    R.op = op;
+    R.ctx = c;
+    layout(c, nrhs, &R.w, (char*)work);
+    R.x = x;
+    R.b = b;
+    R.nrhs = nrhs;
+    R.link_prec = params->precision == LMX_F16 ? LMX_F16 : LMX_F32;
+    R.rhs_block = params->rhs_block > 0 ? params->rhs_block : default_rhs_block(op, params->precision, nrhs);
+    const int V = (int)c->V;
+    dim3 grid(nblocks(c->V), nrhs), blk(256);
+    const int rmax = params->reliable_maxiter > 0 ? params->reliable_maxiter : 100;
+    LMX_HIP(hipMemsetAsync(R.w.counters, 0, nrhs * sizeof(unsigned), c->stream));
+    hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(64), 0, c->stream, R.w.rhs, R.w.batch, nrhs, params->reliable_delta,
+                       rmax);
+    LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
+    hipLaunchKernelGGL(k_mx_init, grid, blk, 0, c->stream, b, R.w.r64, R.w.rlo, R.w.p, R.w.xlo, V, params->rtol,
+                       params->atol, R.w.partials, R.w.counters, R.w.rhs, R.w.batch);
+    LMX_HIP(hipGetLastError());
+
+    const int chunk = params->check_every > 0 ? params->check_every : 10;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (params->use_graph) {
+        hipStream_t cap;
+        LMX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipStream_t saved = c->stream;
+        c->stream = cap;
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
+        int rc = LMX_OK;
+        if (e == hipSuccess) {
+            for (int k = 0; k < chunk && rc == LMX_OK; ++k) rc = enqueue_iteration(R);
+            e = hipStreamEndCapture(cap, &graph);
+        }
+        c->stream = saved;
+        (void)hipStreamDestroy(cap);
+        LMX_TRY(rc);
+        LMX_HIP(e);
+        LMX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    std::vector<MixRhs> hs(nrhs);
+    MixBatch hb;
+    int launched = 0;
+    int rc = LMX_OK;
+    while (true) {
+        LMX_HIP(hipMemcpyAsync(&hb, R.w.batch, sizeof(MixBatch), hipMemcpyDeviceToHost, c->stream));
+        LMX_HIP(hipStreamSynchronize(c->stream));
+        if (hb.n_active <= 0 || launched >= params->maxiter) break;
+        if (exec) {
+            if (hipGraphLaunch(exec, c->stream) != hipSuccess) {
+                rc = LMX_ERR_HIP;
+                set_error("hipGraphLaunch failed");
+                break;
+            }
+            launched += chunk;
+        } else {
+            const int n = std::min(chunk, params->maxiter - launched);
+            for (int k = 0; k < n && rc == LMX_OK; ++k) rc = enqueue_iteration(R);
+            if (rc != LMX_OK) break;
+            launched += n;
+        }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    LMX_TRY(rc);
+    // RHS stopped by maxiter: fold the inner solution in and report the true residual
+    hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, x, R.w.xlo, V, R.w.rhs, R.w.batch, 1);
+    LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
+    hipLaunchKernelGGL(k_mx_true_res, grid, blk, 0, c->stream, b, R.w.r64, R.w.rlo, V, R.w.partials, R.w.counters,
+                       R.w.rhs, R.w.batch, 1);
+    LMX_HIP(hipGetLastError());
+    LMX_HIP(hipMemcpyAsync(hs.data(), R.w.rhs, nrhs * sizeof(MixRhs), hipMemcpyDeviceToHost, c->stream));
+    LMX_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < nrhs; ++i) {
+        const MixRhs& s = hs[i];
+        lmx_cg_result& r = results[i];
+        r.converged = s.converged;
+        r.iterations = s.iterations;
+        r.reason = s.done ? s.reason : LMX_CG_MAXITER;
+        r.reliable_updates = s.updates;
+        r.residual_norm = s.rtrue;
+        r.relative_residual = s.bnorm == 0.0 ? (s.rtrue == 0.0 ? 0.0 : INFINITY) : s.rtrue / s.bnorm;
+        r.true_relative_residual = r.relative_residual;
+    }
+    return LMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: time the inner (low-precision) dslash for every RHS blocking
+// (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,3,4}) and keep
+// the fastest per (volume, precision, nrhs, arch); optionally persisted as
+// text lines "V precision nrhs arch rhs_block us_per_rhs".
+// ---------------------------------------------------------------------------
+static std::string arch_name(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+    std::string a = prop.gcnArchName;
+    size_t c = a.find(':');
+    return c == std::string::npos ? a : a.substr(0, c);
+}
+
+static bool cache_lookup(const char* path, long V, int prec, int nrhs, const std::string& arch, int* block,
+                         double* us) {
+    if (!path) return false;
+    FILE* f = fopen(path, "r");
+    if (!f) return false;
+    char a[64];
+    long v;
+    int p, n, blk;
+    double t;
+    bool found = false;
+    while (fscanf(f, "%ld %d %d %63s %d %lf", &v, &p, &n, a, &blk, &t) == 6)
+        if (v == V && p == prec && n == nrhs && arch == a) {
+            *block = blk;
+            *us = t;
+            found = true;
+        }
+    fclose(f);
+    return found;
 }
 
 }  // namespace lmx
 
-extern "C" int lmx_autotune(lmx_wilson*, int, int, const char*, lmx_tune_result*) {
-    lmx::set_error("autotuner not built yet");
-    return LMX_ERR_UNSUPPORTED;
+extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char* cache_path, lmx_tune_result* out) {
+    using namespace lmx;
+    clear_error();
+    LMX_CHECK(op != nullptr && op->ctx != nullptr && out != nullptr, LMX_ERR_ARG, "bad autotune arguments");
+    LMX_CHECK(precision == LMX_F32 || precision == LMX_F16, LMX_ERR_ARG, "autotune precision must be F32 or F16");
+    LMX_CHECK(nrhs >= 1 && nrhs <= 64, LMX_ERR_ARG, "nrhs must be in [1, 64]");
+    lmx_context* c = op->ctx;
+    const std::string arch = arch_name(c->device);
+    int best = 0;
+    double best_us = 0.0;
+    if (cache_lookup(cache_path, (long)c->V, precision, nrhs, arch, &best, &best_us)) {
+        op->tuned_block[precision][nrhs] = best;
+        out->rhs_block = best;
+        out->block_size = 256;
+        out->ms_per_apply = best_us * 1e-3;
+        return LMX_OK;
+    }
+    LMX_TRY(ensure_lo_links(op));
+    const size_t fb = 12 * (size_t)c->V * nrhs * sizeof(float2);
+    float2 *in = nullptr, *o = nullptr;
+    LMX_HIP(hipMalloc(&in, fb));
+    if (hipMalloc(&o, fb) != hipSuccess) {
+        (void)hipFree(in);
+        set_error("autotune buffer allocation failed");
+        return LMX_ERR_NOMEM;
+    }
+    (void)hipMemsetAsync(in, 0, fb, c->stream);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const void* U = precision == LMX_F16 ? (const void*)op->links16 : (const void*)op->links32;
+    const void* F = op->clover64 ? (precision == LMX_F16 ? (const void*)op->clover16 : (const void*)op->clover32)
+                                 : nullptr;
+    const int per_opts[5] = {1, 2, 4, 6, 12};
+    int rc = LMX_OK;
+    best = 0;
+    best_us = 1e30;
+    for (int pi = 0; pi < 5 && rc == LMX_OK; ++pi) {
+        const int per = per_opts[pi];
+        if (per > nrhs) break;
+        for (int nr = 1; nr <= 4 && rc == LMX_OK; ++nr) {
+            if (nr > per || per % nr != 0) continue;
+            const int blk = (per << 4) | nr;
+            for (int k = 0; k < 2 && rc == LMX_OK; ++k)
+                rc = launch_wilson_lo(c, precision, U, F, op->kappa, op->csw, o, in, nrhs, 0, blk, nullptr);
+            const int reps = 5;
+            (void)hipEventRecord(e0, c->stream);
+            for (int k = 0; k < reps && rc == LMX_OK; ++k)
+                rc = launch_wilson_lo(c, precision, U, F, op->kappa, op->csw, o, in, nrhs, 0, blk, nullptr);
+            (void)hipEventRecord(e1, c->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = LMX_ERR_HIP;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double us = 1e3 * ms / reps / nrhs;
+            if (rc == LMX_OK && us < best_us) {
+                best_us = us;
+                best = blk;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(in);
+    (void)hipFree(o);
+    LMX_TRY(rc);
+    op->tuned_block[precision][nrhs] = best;
+    if (cache_path) {
+        FILE* f = fopen(cache_path, "a");
+        if (f) {
+            fprintf(f, "%ld %d %d %s %d %.3f\n", (long)c->V, precision, nrhs, arch.c_str(), best, best_us);
+            fclose(f);
+        }
+    }
+    out->rhs_block = best;
+    out->block_size = 256;
+    out->ms_per_apply = best_us * 1e-3;
+    return LMX_OK;
 }

@@ -129,6 +129,7 @@ struct EpiArgs {
     CGState* st;
     double* partials;
     unsigned* counter;
+    const int* gate;
 };
 
 enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
@@ -140,6 +141,7 @@ __global__ void __launch_bounds__(256) k_wilson(Out out, In in, Lk U, Cl F, Geom
                                                 long rstride, int ngroups, int nrhs, EpiArgs epi) {
     using R = typename In::R;
     if (epi.st != nullptr && epi.st->done) return;
+    if (epi.gate != nullptr && *epi.gate == 0) return;
     __shared__ double lds4[4];
     __shared__ int lds_flag;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -275,7 +277,7 @@ struct F64Clover {
 int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
                       double2* out, const double2* in, int nrhs, int flags, const Epilogue& e) {
     if (ctx->V == 0 || nrhs == 0) return LMX_OK;
-    EpiArgs epi{e.aux, e.st, e.partials, e.counter};
+    EpiArgs epi{e.aux, e.st, e.partials, e.counter, e.gate};
     const long rstride = 12 * ctx->V;
     const double ccoef = (-kappa) * csw;  // coefficient = -kappa * cSW (:572)
     F64Out o{out};
@@ -317,9 +319,9 @@ struct F16Clover {
 
 template <class Lk, class Cl, int NR>
 static int launch_lo_nr(lmx_context* ctx, Lk U, Cl F, double kappa, double ccoef, float2* out, const float2* in,
-                        int nrhs, int flags, bool clover, int rhs_in_launch, CGState* st) {
+                        int nrhs, int flags, bool clover, int rhs_in_launch, const int* gate) {
     // One launch covers rhs_in_launch right-hand sides split into groups of NR.
-    EpiArgs epi{nullptr, st, nullptr, nullptr};
+    EpiArgs epi{nullptr, nullptr, nullptr, nullptr, gate};
     const long rstride = 12 * ctx->V;
     for (int r0 = 0; r0 < nrhs; r0 += rhs_in_launch) {
         const int n = std::min(rhs_in_launch, nrhs - r0);
@@ -334,20 +336,20 @@ static int launch_lo_nr(lmx_context* ctx, Lk U, Cl F, double kappa, double ccoef
 
 template <class Lk, class Cl>
 static int launch_lo_any(lmx_context* ctx, Lk U, Cl F, double kappa, double ccoef, float2* out, const float2* in,
-                         int nrhs, int flags, bool clover, int rhs_block, CGState* st) {
+                         int nrhs, int flags, bool clover, int rhs_block, const int* gate) {
     // rhs_block encodes (rhs per launch) * 16 + (rhs per work item).
     int per_launch = rhs_block >> 4, nr = rhs_block & 15;
     if (per_launch <= 0) per_launch = 1;
     if (nr <= 0) nr = 1;
     switch (nr) {
     case 1:
-        return launch_lo_nr<Lk, Cl, 1>(ctx, U, F, kappa, ccoef, out, in, nrhs, flags, clover, per_launch, st);
+        return launch_lo_nr<Lk, Cl, 1>(ctx, U, F, kappa, ccoef, out, in, nrhs, flags, clover, per_launch, gate);
     case 2:
-        return launch_lo_nr<Lk, Cl, 2>(ctx, U, F, kappa, ccoef, out, in, nrhs, flags, clover, per_launch, st);
+        return launch_lo_nr<Lk, Cl, 2>(ctx, U, F, kappa, ccoef, out, in, nrhs, flags, clover, per_launch, gate);
     case 3:
-        return launch_lo_nr<Lk, Cl, 3>(ctx, U, F, kappa, ccoef, out, in, nrhs, flags, clover, per_launch, st);
+        return launch_lo_nr<Lk, Cl, 3>(ctx, U, F, kappa, ccoef, out, in, nrhs, flags, clover, per_launch, gate);
     case 4:
-        return launch_lo_nr<Lk, Cl, 4>(ctx, U, F, kappa, ccoef, out, in, nrhs, flags, clover, per_launch, st);
+        return launch_lo_nr<Lk, Cl, 4>(ctx, U, F, kappa, ccoef, out, in, nrhs, flags, clover, per_launch, gate);
     default:
         set_error("unsupported rhs register block %d", nr);
         return LMX_ERR_UNSUPPORTED;
@@ -355,17 +357,17 @@ static int launch_lo_any(lmx_context* ctx, Lk U, Cl F, double kappa, double ccoe
 }
 
 int launch_wilson_lo(lmx_context* ctx, int link_prec, const void* U, const void* clover, double kappa, double csw,
-                     float2* out, const float2* in, int nrhs, int flags, int rhs_block, CGState* st) {
+                     float2* out, const float2* in, int nrhs, int flags, int rhs_block, const int* gate) {
     if (ctx->V == 0 || nrhs == 0) return LMX_OK;
     const double ccoef = (-kappa) * csw;
     const bool cl = clover != nullptr;
     if (link_prec == LMX_F32) {
         return launch_lo_any(ctx, F32In{(const float2*)U}, CloverWrap<F32Clover>{F32Clover{(const float2*)clover}},
-                             kappa, ccoef, out, in, nrhs, flags, cl, rhs_block, st);
+                             kappa, ccoef, out, in, nrhs, flags, cl, rhs_block, gate);
     }
     if (link_prec == LMX_F16) {
         return launch_lo_any(ctx, F16In{(const __half2*)U}, CloverWrap<F16Clover>{F16Clover{(const __half2*)clover}},
-                             kappa, ccoef, out, in, nrhs, flags, cl, rhs_block, st);
+                             kappa, ccoef, out, in, nrhs, flags, cl, rhs_block, gate);
     }
     set_error("unsupported link precision %d", link_prec);
     return LMX_ERR_UNSUPPORTED;

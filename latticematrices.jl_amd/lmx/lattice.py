@@ -451,3 +451,48 @@ def wilson_pullback_(dU: GaugeField, D: WilsonDiracOperator4D, dresult: SpinorFi
 def ta_project_(F: GaugeField, U: GaugeField, G: GaugeField) -> GaugeField:
     check(lib().lmx_ta_project(U.lattice.ctx, F.ptr, U.ptr, G.ptr))
     return F
+
+
+_PREC = {"f64": _lib.LMX_F64, "f32": _lib.LMX_F32, "f16": _lib.LMX_F16}
+
+
+def cg_batched_(x: SpinorField, D: WilsonDiracOperator4D, b: SpinorField, precision: str = "f32", rtol=1e-10,
+                atol=0.0, maxiter=20000, reliable_delta=0.1, reliable_maxiter=100, check_every=0,
+                use_graph=False, rhs_block=0, workspace=None) -> list:
+    """Solve D^dagger D x_i = b_i for the x.nrhs right-hand sides at once.
+
+    precision: "f64" (plain cg! per RHS), "f32" (FP32 inner CG) or "f16"
+    (FP16-stored links, FP32 spinors and math); the mixed modes use FP64
+    reliable updates, FP64 reductions and FP64 solution accumulation, and
+    stop on the FP64 true residual.  x holds the initial guess.  rhs_block
+    encodes (RHS per launch) << 4 | (RHS per work item); 0 uses the
+    autotuned or default shape.  Returns one CGResult per RHS."""
+    if x.nrhs != b.nrhs:
+        raise ArgumentError(_lib.LMX_ERR_ARG, "x and b must hold the same number of right-hand sides")
+    if x.data.data_ptr() == b.data.data_ptr():
+        raise ArgumentError(_lib.LMX_ERR_ALIAS, "CG fields x and b must not alias")
+    params = _lib.default_params(rtol=float(rtol), atol=float(atol), maxiter=int(maxiter),
+                                 precision=_PREC[precision], reliable_delta=float(reliable_delta),
+                                 reliable_maxiter=int(reliable_maxiter), check_every=int(check_every),
+                                 use_graph=int(bool(use_graph)), rhs_block=int(rhs_block))
+    nbytes = ctypes.c_size_t()
+    check(lib().lmx_cg_batched_workspace(D._op, x.nrhs, ctypes.byref(params), ctypes.byref(nbytes)))
+    if workspace is None or workspace.numel() < nbytes.value:
+        key = (x.nrhs, precision)
+        cache = D.__dict__.setdefault("_ws", {})
+        workspace = cache.get(key)
+        if workspace is None or workspace.numel() < nbytes.value:
+            workspace = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=x.lattice.device)
+            cache[key] = workspace
+    res = (_lib.CGResultC * x.nrhs)()
+    check(lib().lmx_cg_batched(D._op, x.ptr, b.ptr, x.nrhs, ctypes.c_void_p(workspace.data_ptr()),
+                               nbytes.value, ctypes.byref(params), res))
+    return [CGResult.from_c(r) for r in res]
+
+
+def autotune(D: WilsonDiracOperator4D, precision: str, nrhs: int, cache_path: str | None = None):
+    """Pick the RHS blocking of the mixed solver's dslash (lmx_autotune)."""
+    out = _lib.TuneResult()
+    check(lib().lmx_autotune(D._op, _PREC[precision], int(nrhs),
+                             cache_path.encode() if cache_path else None, ctypes.byref(out)))
+    return {"rhs_block": int(out.rhs_block), "block_size": int(out.block_size), "ms_per_apply": out.ms_per_apply}

@@ -1,0 +1,126 @@
+"""Mixed-precision and multi-RHS D^dagger D CG on the GPU against the FP64
+oracle.  The mixed solvers stop on the FP64 true residual, so the bar is
+|b - A x| / |b| <= rtol (checked independently on the host with the oracle's
+operator) and the solution error against the oracle's FP64 solution, which
+is bounded by cond(A) * rtol; both are printed per RHS."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import lmx
+from lmx import generators as gen
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(L, kind, seed, kappa, csw=0.0, nrhs=3):
+    U = gen.links(L, kind, seed, eps=0.2)
+    b = gen.gaussian_spinor(L, seed + 1, nrhs=nrhs)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracOperator4D(Ug, kappa, csw)
+    return U, b, lat, Ug, D
+
+
+def oracle_check(U, L, kappa, b, x, rtol, csw=0.0):
+    F = orc.clover_field_strength(U, L) if csw else None
+    op = orc.NormalOp(orc.OP_DDAGD_CLOVER if csw else orc.OP_DDAGD_WILSON, L, U=U, F=F, kappa=kappa, csw=csw)
+    out = []
+    for i in range(b.shape[0]):
+        res = np.linalg.norm(op.apply(x[i]) - b[i]) / np.linalg.norm(b[i])
+        xo, ro = orc.cg(op, b[i], rtol=1e-12, maxiter=20000)
+        err = np.linalg.norm(x[i] - xo) / np.linalg.norm(xo)
+        out.append((res, err))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16"])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_mixed_batched_converges_to_fp64_true_residual(gpu, precision, use_graph):
+    L, kappa, rtol = (8, 8, 8, 8), 0.124, 1e-10
+    U, b, lat, Ug, D = setup(L, "weak", 300, kappa)
+    print(f"mixed {precision} graph={use_graph} L={L}", flush=True)
+    x = lat.spinor(3)
+    bg = lat.spinor_from_numpy(b)
+    res = lmx.cg_batched_(x, D, bg, precision=precision, rtol=rtol, use_graph=use_graph)
+    xs = x.to_numpy()
+    checks = oracle_check(U, L, kappa, b, xs, rtol)
+    for r, (tr, err) in zip(res, checks):
+        print(f"  iters={r.iterations} updates={r.reliable_updates} true_rel={tr:.3e} "
+              f"reported={r.true_relative_residual:.3e} err_vs_fp64={err:.3e}", flush=True)
+        assert r.converged and r.reason == "converged"
+        assert tr <= rtol * 1.05
+        assert abs(tr - r.true_relative_residual) <= 1e-3 * rtol + 1e-6 * tr
+        assert err <= 1e-7
+
+
+def test_graph_and_eager_agree(gpu):
+    L, kappa = (8, 4, 4, 8), 0.12
+    U, b, lat, Ug, D = setup(L, "haar", 310, kappa, nrhs=2)
+    bg = lat.spinor_from_numpy(b)
+    x1, x2 = lat.spinor(2), lat.spinor(2)
+    r1 = lmx.cg_batched_(x1, D, bg, precision="f32", use_graph=False, check_every=5)
+    r2 = lmx.cg_batched_(x2, D, bg, precision="f32", use_graph=True, check_every=5)
+    assert [r.iterations for r in r1] == [r.iterations for r in r2]
+    assert np.array_equal(x1.to_numpy(), x2.to_numpy())
+
+
+@pytest.mark.parametrize("rhs_block", [(1 << 4) | 1, (4 << 4) | 2, (5 << 4) | 1, (3 << 4) | 3, (4 << 4) | 4])
+def test_rhs_blocking_uneven_batch(gpu, rhs_block):
+    # 5 RHS split unevenly by the launch shape; every shape gives the same iterates
+    L, kappa = (4, 4, 4, 8), 0.12
+    U, b, lat, Ug, D = setup(L, "haar", 320, kappa, nrhs=5)
+    bg = lat.spinor_from_numpy(b)
+    x = lat.spinor(5)
+    res = lmx.cg_batched_(x, D, bg, precision="f16", rhs_block=rhs_block)
+    ref = lat.spinor(5)
+    rref = lmx.cg_batched_(ref, D, bg, precision="f16", rhs_block=(1 << 4) | 1)
+    assert [r.iterations for r in res] == [r.iterations for r in rref]
+    assert np.array_equal(x.to_numpy(), ref.to_numpy())
+    assert all(r.converged for r in res)
+
+
+def test_plain_fp64_batched_matches_single(gpu):
+    L, kappa = (4, 4, 4, 4), 0.125
+    U, b, lat, Ug, D = setup(L, "haar", 330, kappa, nrhs=2)
+    bg = lat.spinor_from_numpy(b)
+    x = lat.spinor(2)
+    res = lmx.cg_batched_(x, D, bg, precision="f64", rtol=1e-10)
+    op = orc.NormalOp(orc.OP_DDAGD_WILSON, L, U=U, kappa=kappa)
+    for i in range(2):
+        xo, ro = orc.cg(op, b[i], rtol=1e-10, order=1)
+        assert res[i].iterations == ro.iterations
+        assert np.array_equal(x.to_numpy()[i], xo)
+
+
+def test_mixed_edge_cases(gpu):
+    L, kappa = (4, 4, 4, 4), 0.12
+    U, b, lat, Ug, D = setup(L, "haar", 340, kappa, nrhs=3)
+    b[1] = 0.0  # a zero right-hand side converges at iteration 0
+    bg = lat.spinor_from_numpy(b)
+    x = lat.spinor(3)
+    res = lmx.cg_batched_(x, D, bg, precision="f32")
+    assert res[1].converged and res[1].iterations == 0 and res[1].residual_norm == 0.0
+    assert res[0].converged and res[2].converged
+    assert np.all(x.to_numpy()[1] == 0)
+    # maxiter reached: status, not an exception
+    x2 = lat.spinor(3)
+    res = lmx.cg_batched_(x2, D, bg, precision="f32", maxiter=3, check_every=1)
+    assert res[0].reason == "maximum_iterations" and not res[0].converged
+    with pytest.raises(lmx.ArgumentError):
+        lmx.cg_batched_(x2, D, x2, precision="f32")
+
+
+def test_mixed_clover(gpu):
+    L, kappa, csw, rtol = (8, 4, 4, 8), 0.1235, 1.0, 1e-10
+    U = gen.links(L, "weak", 350, eps=0.15)
+    b = gen.gaussian_spinor(L, 351, nrhs=2)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    D = lmx.WilsonDiracCloverOperator4D(lat.gauge_from_numpy(U), kappa, csw)
+    bg = lat.spinor_from_numpy(b)
+    x = lat.spinor(2)
+    res = lmx.cg_batched_(x, D, bg, precision="f16", rtol=rtol)
+    checks = oracle_check(U, L, kappa, b, x.to_numpy(), rtol, csw=csw)
+    for r, (tr, err) in zip(res, checks):
+        print(f"  clover iters={r.iterations} true_rel={tr:.3e} err={err:.3e}", flush=True)
+        assert r.converged and tr <= rtol * 1.05 and err <= 1e-7
