@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--mode", type=str, default="full", choices=["full", "kernels"],
+                    help="kernels: only the dslash / copy kernel timings (for profiler counter runs)")
     return ap.parse_args()
 
 
@@ -132,6 +134,16 @@ def main():
     log(f"bench: FP64 D_W {t_d64 * 1e3:.1f} us/launch, {ach_d64:.0f} GB/s algorithmic")
     t_copy = copy_calibration(lat)
     log(f"bench: 1 GiB copy {t_copy * 1e3:.1f} us ({2 * (1 << 30) / (t_copy * 1e-3) / 1e9:.0f} GB/s)")
+
+    if args.mode == "kernels":
+        tune = lmx.autotune(D, args.precision, n, None)
+        bytes_dm = (144.0 if args.precision == "f16" else 288.0) / n + 192.0
+        log(f"bench: {args.precision} dslash x{n}: {tune['ms_per_apply'] * 1e3:.1f} us/RHS, "
+            f"{bytes_dm * V / (tune['ms_per_apply'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
+        print(json.dumps({"mode": "kernels", "fp64_dslash_us": t_d64 * 1e3, "fp64_dslash_gbs": ach_d64,
+                          "copy_1gib_us": t_copy * 1e3, "lo_dslash_us_per_rhs": tune["ms_per_apply"] * 1e3,
+                          "lo_rhs_block": tune["rhs_block"]}), flush=True)
+        return
 
     # ---- plain FP64 CG (the reference algorithm), RHS 0 -------------------
     def solve64():

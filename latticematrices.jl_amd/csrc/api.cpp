@@ -159,7 +159,11 @@ int lmx_context_create(lmx_context** out, const int32_t dims[4], const double* p
         g.iph_re[d] = re / dd;
         g.iph_im[d] = -im / dd;
         g.phase_one[d] = (re == 1.0 && im == 0.0) ? 1 : 0;
+        g.phase_neg[d] = (re == -1.0 && im == 0.0) ? 1 : 0;
     }
+    g.general = 0;
+    for (int d = 0; d < 4; ++d)
+        if (!g.phase_one[d] && !g.phase_neg[d]) g.general = 1;
     g.V = (int)V;
     c->V = V;
     c->stream = (hipStream_t)stream;

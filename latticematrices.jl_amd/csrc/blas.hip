@@ -11,7 +11,6 @@
 
 namespace lmx {
 
-enum { FIN_OUT = 0, FIN_BB = 1, FIN_RR0 = 2, FIN_RRNEW = 3, FIN_PAP = 4 };
 
 __device__ __forceinline__ void finalize_scalar(int kind, CGState* st, double re, double im, double* out2) {
     switch (kind) {
@@ -90,37 +89,39 @@ __device__ __forceinline__ void finalize_scalar(int kind, CGState* st, double re
     }
 }
 
-// Shared tail: reduce (re, im) over the grid and finalize in the last block.
-__device__ __forceinline__ void grid_reduce(double re, double im, bool want_im, int lb, double* partials,
-                                            unsigned* counter, int kind, CGState* st, double* out2) {
+// Block partials of (re, im) over the block's 256 sites (fixed tree).
+__device__ __forceinline__ void store_partials(double re, double im, bool want_im, int lb, double* partials) {
     __shared__ double lds4[4];
-    __shared__ int flag;
     const int nb = gridDim.x;
     double tre = block_tree(re, lds4);
     double tim = want_im ? block_tree(im, lds4) : 0.0;
     if (threadIdx.x == 0) {
         partials[lb] = tre;
         if (want_im) partials[nb + lb] = tim;
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        flag = (prev == (unsigned)nb - 1) ? 1 : 0;
     }
-    __syncthreads();
-    if (flag) {
-        __threadfence();
-        double fre = finish_partials(partials, nb, lds4);
-        double fim = want_im ? finish_partials(partials + nb, nb, lds4) : 0.0;
-        if (threadIdx.x == 0) {
-            finalize_scalar(kind, st, fre, fim, out2);
-            *counter = 0u;
-        }
-    }
+}
+
+// Second stage of every reduction: one block sums the partials in a fixed
+// order and updates the device CG state (or writes the scalar out).
+__global__ void __launch_bounds__(256) k_finalize(const double* __restrict__ partials, int nb, int kind, int want_im,
+                                                  CGState* st, double* out2) {
+    if (st != nullptr && st->done) return;
+    __shared__ double lds4[4];
+    double fre = finish_partials(partials, nb, lds4);
+    double fim = want_im ? finish_partials(partials + nb, nb, lds4) : 0.0;
+    if (threadIdx.x == 0) finalize_scalar(kind, st, fre, fim, out2);
+}
+
+int launch_finalize(lmx_context* ctx, int nb, int kind, bool want_im, CGState* st, double* out2) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partials, nb, kind, want_im ? 1 : 0, st,
+                       out2);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
 }
 
 // dot(A, B) = sum conj(A) B
 __global__ void __launch_bounds__(256) k_dot(const double2* __restrict__ A, const double2* __restrict__ B, int V,
-                                             double* partials, unsigned* counter, int kind, CGState* st,
-                                             double* out2) {
+                                             double* partials, int want_im, CGState* st) {
     if (st != nullptr && st->done) return;
     const int lb = blockIdx.x;
     const int s = lb * 256 + threadIdx.x;
@@ -135,7 +136,7 @@ __global__ void __launch_bounds__(256) k_dot(const double2* __restrict__ A, cons
         re = acc.re;
         im = acc.im;
     }
-    grid_reduce(re, im, kind == FIN_OUT, lb, partials, counter, kind, st, out2);
+    store_partials(re, im, want_im != 0, lb, partials);
 }
 
 __global__ void __launch_bounds__(256) k_axpby(double a, const double* __restrict__ X, double b, double* __restrict__ Y,
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(256) k_scale(double2* __restrict__ out, const 
 // r = b - Ap (axpby!(1,b,0,r); axpby!(-1,Ap,1,r)); p = r; rr = |r|^2  (cg.jl:91-95)
 __global__ void __launch_bounds__(256) k_cg_setup_r(const double2* __restrict__ b, const double2* __restrict__ Ap,
                                                     double2* __restrict__ r, double2* __restrict__ p, int V,
-                                                    double* partials, unsigned* counter, CGState* st) {
+                                                    double* partials, CGState* st) {
     if (st->done) return;
     const int lb = blockIdx.x;
     const int s = lb * 256 + threadIdx.x;
@@ -176,13 +177,13 @@ __global__ void __launch_bounds__(256) k_cg_setup_r(const double2* __restrict__ 
         }
         re = acc.re;
     }
-    grid_reduce(re, 0.0, false, lb, partials, counter, FIN_RR0, st, nullptr);
+    store_partials(re, 0.0, false, lb, partials);
 }
 
 // x = alpha*p + x; r = (-alpha)*Ap + r; rr_new = |r|^2  (cg.jl:116-120)
 __global__ void __launch_bounds__(256) k_cg_update_xr(double2* __restrict__ x, double2* __restrict__ r,
                                                       const double2* __restrict__ p, const double2* __restrict__ Ap,
-                                                      int V, double* partials, unsigned* counter, CGState* st) {
+                                                      int V, double* partials, CGState* st) {
     if (st->done) return;
     const double alpha = st->alpha;
     const double malpha = -alpha;
@@ -204,7 +205,7 @@ __global__ void __launch_bounds__(256) k_cg_update_xr(double2* __restrict__ x, d
         }
         re = acc.re;
     }
-    grid_reduce(re, 0.0, false, lb, partials, counter, FIN_RRNEW, st, nullptr);
+    store_partials(re, 0.0, false, lb, partials);
 }
 
 // p = 1*r + beta*p  (cg.jl:130-131)
@@ -243,10 +244,11 @@ static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)
 
 int launch_dot(lmx_context* ctx, const double2* A, const double2* B, int fin_kind, CGState* st, double* out2) {
     const int nb = nblocks(ctx->V);
-    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(256), 0, ctx->stream, A, B, (int)ctx->V, ctx->partials,
-                       ctx->counters + 0, fin_kind, st, out2);
+    const bool want_im = fin_kind == FIN_OUT;
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(256), 0, ctx->stream, A, B, (int)ctx->V, ctx->partials, want_im ? 1 : 0,
+                       st);
     LMX_HIP(hipGetLastError());
-    return LMX_OK;
+    return launch_finalize(ctx, nb, fin_kind, want_im, st, out2);
 }
 
 int launch_axpby(lmx_context* ctx, double a, const double2* X, double b, double2* Y, int64_t n_complex) {
@@ -266,16 +268,16 @@ int launch_scale(lmx_context* ctx, double2* out, const double2* in, double sr, d
 
 int launch_cg_setup_r(lmx_context* ctx, const double2* b, const double2* Ap, double2* r, double2* p, CGState* st) {
     hipLaunchKernelGGL(k_cg_setup_r, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, b, Ap, r, p, (int)ctx->V,
-                       ctx->partials, ctx->counters + 0, st);
+                       ctx->partials, st);
     LMX_HIP(hipGetLastError());
-    return LMX_OK;
+    return launch_finalize(ctx, nblocks(ctx->V), FIN_RR0, false, st, nullptr);
 }
 
 int launch_cg_update_xr(lmx_context* ctx, double2* x, double2* r, const double2* p, const double2* Ap, CGState* st) {
     hipLaunchKernelGGL(k_cg_update_xr, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, x, r, p, Ap, (int)ctx->V,
-                       ctx->partials, ctx->counters + 0, st);
+                       ctx->partials, st);
     LMX_HIP(hipGetLastError());
-    return LMX_OK;
+    return launch_finalize(ctx, nblocks(ctx->V), FIN_RRNEW, false, st, nullptr);
 }
 
 int launch_cg_update_p(lmx_context* ctx, const double2* r, double2* p, CGState* st) {

@@ -76,6 +76,8 @@ struct Geom {
     int stride[4];
     int V;
     int phase_one[4];  // phase == 1 exactly: the wrap multiply is skipped (as _mul_phase!)
+    int phase_neg[4];  // phase == -1 exactly: the wrap is an exact sign flip
+    int general;       // some phase is neither 1 nor -1 (complex multiply on wrap)
     double ph_re[4], ph_im[4];
     double iph_re[4], iph_im[4];
 };
@@ -218,6 +220,19 @@ __device__ __forceinline__ void matvec_bwd(const cx<R> u[9], const cx<R> h[6], c
     }
 }
 
+// Negate a half-spinor where `neg` holds (a boundary phase of -1).  Negation
+// commutes exactly with projection and the link product, so flipping the
+// projected half-spinor is bit-identical to flipping the neighbour spinor as
+// the halo fill does; selects keep the stencil free of divergent branches.
+template <typename R>
+__device__ __forceinline__ void neg_select(cx<R> h[6], bool neg) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        h[k].re = neg ? -h[k].re : h[k].re;
+        h[k].im = neg ? -h[k].im : h[k].im;
+    }
+}
+
 // Multiply a full spinor by a boundary phase (the halo fill's _mul_phase!).
 template <typename R>
 __device__ __forceinline__ void phase_spinor(cx<R> psi[12], double pre, double pim) {
@@ -255,29 +270,13 @@ __device__ __forceinline__ double block_tree(double v, double* lds4) {
     return t;
 }
 
-// Second stage: 256 lanes sum the block partials with stride 256 in order,
-// then the block tree.  Called by every thread of the last block.
+// Second stage (its own one-block launch, after the producing kernel has
+// finished, so every partial is visible without a per-block fence): 256
+// lanes sum the block partials with stride 256 in order, then the block tree.
 __device__ __forceinline__ double finish_partials(const double* partials, int nb, double* lds4) {
     double acc = 0.0;
     for (int b = threadIdx.x; b < nb; b += kRedBlock) acc = acc + partials[b];
     return block_tree(acc, lds4);
-}
-
-// Publish a block partial; returns true in every thread of the block that
-// completes the grid (the "last block"), after which the partials of all
-// blocks are visible to it.
-__device__ __forceinline__ bool publish_partial(double total, double* partials, int lb, unsigned* counter,
-                                                unsigned nblocks, int* lds_flag) {
-    if (threadIdx.x == 0) {
-        partials[lb] = total;
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        *lds_flag = (prev == nblocks - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    bool last = *lds_flag != 0;
-    if (last) __threadfence();
-    return last;
 }
 
 }  // namespace lmx

@@ -112,6 +112,8 @@ int launch_convert_links(lmx_context* ctx, const double2* U, float2* U32, __half
 int launch_convert_clover(lmx_context* ctx, const double2* F, float2* F32, __half2* F16);
 
 // ---- BLAS / CG kernels (blas.hip) ----
+enum { FIN_OUT = 0, FIN_BB = 1, FIN_RR0 = 2, FIN_RRNEW = 3, FIN_PAP = 4 };
+int launch_finalize(lmx_context* ctx, int nb, int kind, bool want_im, CGState* st, double* out2);
 int launch_dot(lmx_context* ctx, const double2* A, const double2* B, int fin_kind, CGState* st, double* out2);
 int launch_axpby(lmx_context* ctx, double a, const double2* X, double b, double2* Y, int64_t n);
 int launch_scale(lmx_context* ctx, double2* out, const double2* in, double sr, double si, int64_t n);

@@ -45,7 +45,6 @@ struct MixPtrs {
     double2* r64;
     double2* t64;
     double* partials;
-    unsigned* counters;
     MixRhs* rhs;
     MixBatch* batch;
 };
@@ -71,7 +70,6 @@ static size_t layout(lmx_context* ctx, int nrhs, MixPtrs* P, char* base) {
     q.r64 = (double2*)take(f64);
     q.t64 = (double2*)take(f64);
     q.partials = (double*)take(2 * nb * nrhs * sizeof(double));
-    q.counters = (unsigned*)take(nrhs * sizeof(unsigned));
     q.rhs = (MixRhs*)take(nrhs * sizeof(MixRhs));
     q.batch = (MixBatch*)take(sizeof(MixBatch));
     if (P) *P = q;
@@ -79,34 +77,18 @@ static size_t layout(lmx_context* ctx, int nrhs, MixPtrs* P, char* base) {
 }
 
 // ---------------------------------------------------------------------------
-// Reduction helper: every block of row i (blockIdx.y) reduces (a, b) over its
-// sites; the last block of the row calls fin(i, A, B) in thread 0.
+// Reductions: every block of row i (blockIdx.y = RHS) stores the block sums
+// of (a, b) over its 256 sites; a separate one-block-per-RHS finalize launch
+// sums them in a fixed order and updates that RHS's state.
 // ---------------------------------------------------------------------------
-template <class Fin>
-__device__ __forceinline__ void row_reduce(double a, double b, double* partials, unsigned* counters, Fin fin) {
+__device__ __forceinline__ void row_partials(double a, double b, double* partials) {
     __shared__ double lds4[4];
-    __shared__ int flag;
     const int nb = gridDim.x, i = blockIdx.y, bx = blockIdx.x;
     double ta = block_tree(a, lds4);
     double tb = block_tree(b, lds4);
-    double* pa = partials + (size_t)(2 * i) * nb;
-    double* pb = partials + (size_t)(2 * i + 1) * nb;
     if (threadIdx.x == 0) {
-        pa[bx] = ta;
-        pb[bx] = tb;
-        __threadfence();
-        unsigned prev = atomicAdd(counters + i, 1u);
-        flag = (prev == (unsigned)nb - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (flag) {
-        __threadfence();
-        double A = finish_partials(pa, nb, lds4);
-        double B = finish_partials(pb, nb, lds4);
-        if (threadIdx.x == 0) {
-            fin(i, A, B);
-            counters[i] = 0u;
-        }
+        partials[(size_t)(2 * i) * nb + bx] = ta;
+        partials[(size_t)(2 * i + 1) * nb + bx] = tb;
     }
 }
 
@@ -117,12 +99,77 @@ __device__ __forceinline__ void retire(MixRhs* st, MixBatch* batch, int reason, 
     atomicSub(&batch->n_active, 1);
 }
 
+enum { MX_INIT = 0, MX_PAP = 1, MX_RR = 2, MX_TRUE = 3 };
+
+template <int KIND>
+__global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ partials, int nb, MixRhs* rhs,
+                                                MixBatch* batch, double rtol, double atol, int force) {
+    const int i = blockIdx.x;
+    MixRhs* st = rhs + i;
+    if (KIND == MX_PAP && i == 0 && threadIdx.x == 0) batch->do_update = 0;
+    if (KIND == MX_TRUE && !force && !batch->do_update) return;
+    if (st->done) return;
+    __shared__ double lds4[4];
+    const double A = finish_partials(partials + (size_t)(2 * i) * nb, nb, lds4);
+    const double B = (KIND == MX_INIT) ? finish_partials(partials + (size_t)(2 * i + 1) * nb, nb, lds4) : 0.0;
+    if (threadIdx.x != 0) return;
+    if (KIND == MX_INIT) {  // A = |r|^2, B = |b|^2
+        st->bnorm = sqrt(B);
+        st->tol = fmax(atol, rtol * st->bnorm);
+        st->rr = A;
+        st->rr_prev = A;
+        st->rtrue = sqrt(A);
+        st->rref = st->rtrue;
+        st->since = 0;
+        st->iterations = 0;
+        st->updates = 0;
+        st->beta = 0.0;
+        if (!isfinite(A) || !isfinite(B))
+            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+        else if (st->rtrue <= st->tol)
+            retire(st, batch, LMX_CG_CONVERGED, 1);
+    } else if (KIND == MX_PAP) {  // A = Re <p, Ap>
+        st->pAp = A;
+        if (!isfinite(A))
+            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+        else if (A <= 0.0)
+            retire(st, batch, LMX_CG_NONPOSITIVE_CURVATURE, 0);
+        else
+            st->alpha = st->rr / A;
+    } else if (KIND == MX_RR) {  // A = |r_inner|^2
+        st->iterations += 1;
+        st->since += 1;
+        if (!isfinite(A)) {
+            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+            return;
+        }
+        const double rn = sqrt(A);
+        st->rref = fmax(st->rref, rn);
+        st->rr_prev = st->rr;
+        st->beta = A / st->rr;
+        st->rr = A;
+        if (rn < batch->delta * st->rref || st->since >= batch->rmax || rn <= st->tol) atomicOr(&batch->do_update, 1);
+    } else {  // MX_TRUE: A = |b - A x|^2 in FP64
+        st->rtrue = sqrt(A);
+        st->updates += 1;
+        if (!isfinite(A)) {
+            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+        } else if (st->rtrue <= st->tol) {
+            retire(st, batch, LMX_CG_CONVERGED, 1);
+        } else {
+            st->beta = A / st->rr_prev;
+            st->rr = A;
+            st->rref = st->rtrue;
+            st->since = 0;
+        }
+    }
+}
+
 // r64 = b - A x (A x already in r64); r_lo = r64; p = r_lo; x_lo = 0;
-// |r|^2 and |b|^2 per RHS -> initial state.
+// |r|^2 and |b|^2 per RHS.
 __global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, double2* __restrict__ r64,
                                                  float2* __restrict__ rlo, float2* __restrict__ p,
-                                                 float2* __restrict__ xlo, int V, double rtol, double atol,
-                                                 double* partials, unsigned* counters, MixRhs* rhs, MixBatch* batch) {
+                                                 float2* __restrict__ xlo, int V, double* partials) {
     const int i = blockIdx.y;
     const int s = blockIdx.x * 256 + threadIdx.x;
     double rr = 0.0, bb = 0.0;
@@ -142,24 +189,7 @@ __global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, 
             bb += bv.x * bv.x + bv.y * bv.y;
         }
     }
-    row_reduce(rr, bb, partials, counters, [&](int i, double RR, double BB) {
-        MixRhs* st = rhs + i;
-        st->bnorm = sqrt(BB);
-        st->tol = fmax(atol, rtol * st->bnorm);
-        st->rr = RR;
-        st->rr_prev = RR;
-        st->rtrue = sqrt(RR);
-        st->rref = st->rtrue;
-        st->since = 0;
-        st->iterations = 0;
-        st->updates = 0;
-        st->beta = 0.0;
-        if (!isfinite(RR) || !isfinite(BB)) {
-            retire(st, batch, LMX_CG_BREAKDOWN, 0);
-        } else if (st->rtrue <= st->tol) {
-            retire(st, batch, LMX_CG_CONVERGED, 1);
-        }
-    });
+    row_partials(rr, bb, partials);
 }
 
 __global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta, int rmax) {
@@ -178,12 +208,10 @@ __global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta,
     }
 }
 
-// pAp_i = Re <p_i, Ap_i> (FP64 accumulation) -> alpha_i
+// pAp_i = Re <p_i, Ap_i> (FP64 accumulation)
 __global__ void __launch_bounds__(256) k_mx_pap(const float2* __restrict__ p, const float2* __restrict__ Ap, int V,
-                                                double* partials, unsigned* counters, MixRhs* rhs,
-                                                MixBatch* batch) {
+                                                double* partials, const MixRhs* rhs) {
     const int i = blockIdx.y;
-    if (blockIdx.x == 0 && i == 0 && threadIdx.x == 0) batch->do_update = 0;
     if (rhs[i].done) return;
     const int s = blockIdx.x * 256 + threadIdx.x;
     double v = 0.0;
@@ -195,24 +223,13 @@ __global__ void __launch_bounds__(256) k_mx_pap(const float2* __restrict__ p, co
             v += (double)a.x * (double)c.x + (double)a.y * (double)c.y;
         }
     }
-    row_reduce(v, 0.0, partials, counters, [&](int i, double pAp, double) {
-        MixRhs* st = rhs + i;
-        st->pAp = pAp;
-        if (!isfinite(pAp)) {
-            retire(st, batch, LMX_CG_BREAKDOWN, 0);
-        } else if (pAp <= 0.0) {
-            retire(st, batch, LMX_CG_NONPOSITIVE_CURVATURE, 0);
-        } else {
-            st->alpha = st->rr / pAp;
-        }
-    });
+    row_partials(v, 0.0, partials);
 }
 
-// x_lo += alpha p; r_lo -= alpha Ap; |r_lo|^2 -> beta, reliable-update trigger
+// x_lo += alpha p; r_lo -= alpha Ap; |r_lo|^2
 __global__ void __launch_bounds__(256) k_mx_update_xr(float2* __restrict__ xlo, float2* __restrict__ rlo,
                                                       const float2* __restrict__ p, const float2* __restrict__ Ap,
-                                                      int V, double* partials, unsigned* counters, MixRhs* rhs,
-                                                      MixBatch* batch) {
+                                                      int V, double* partials, const MixRhs* rhs) {
     const int i = blockIdx.y;
     if (rhs[i].done) return;
     const float a = (float)rhs[i].alpha;
@@ -233,22 +250,7 @@ __global__ void __launch_bounds__(256) k_mx_update_xr(float2* __restrict__ xlo, 
             v += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
         }
     }
-    row_reduce(v, 0.0, partials, counters, [&](int i, double rr_new, double) {
-        MixRhs* st = rhs + i;
-        st->iterations += 1;
-        st->since += 1;
-        if (!isfinite(rr_new)) {
-            retire(st, batch, LMX_CG_BREAKDOWN, 0);
-            return;
-        }
-        const double rn = sqrt(rr_new);
-        st->rref = fmax(st->rref, rn);
-        st->rr_prev = st->rr;
-        st->beta = rr_new / st->rr;
-        st->rr = rr_new;
-        const bool trigger = rn < batch->delta * st->rref || st->since >= batch->rmax || rn <= st->tol;
-        if (trigger) atomicOr(&batch->do_update, 1);
-    });
+    row_partials(v, 0.0, partials);
 }
 
 // reliable update, part 1: x += x_lo (FP64), x_lo = 0
@@ -269,10 +271,10 @@ __global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float
     }
 }
 
-// reliable update, part 2: r64 = b - A x (A x in r64), r_lo = r64, true |r|
+// reliable update, part 2: r64 = b - A x (A x in r64), r_lo = r64, |r|^2 in FP64
 __global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__ b, double2* __restrict__ r64,
                                                      float2* __restrict__ rlo, int V, double* partials,
-                                                     unsigned* counters, MixRhs* rhs, MixBatch* batch, int force) {
+                                                     const MixRhs* rhs, const MixBatch* batch, int force) {
     const int i = blockIdx.y;
     if ((!force && !batch->do_update) || rhs[i].done) return;
     const int s = blockIdx.x * 256 + threadIdx.x;
@@ -289,21 +291,7 @@ __global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__
             v += rv.x * rv.x + rv.y * rv.y;
         }
     }
-    row_reduce(v, 0.0, partials, counters, [&](int i, double rr, double) {
-        MixRhs* st = rhs + i;
-        st->rtrue = sqrt(rr);
-        st->updates += 1;
-        if (!isfinite(rr)) {
-            retire(st, batch, LMX_CG_BREAKDOWN, 0);
-        } else if (st->rtrue <= st->tol) {
-            retire(st, batch, LMX_CG_CONVERGED, 1);
-        } else {
-            st->beta = rr / st->rr_prev;
-            st->rr = rr;
-            st->rref = st->rtrue;
-            st->since = 0;
-        }
-    });
+    row_partials(v, 0.0, partials);
 }
 
 // p = r_lo + beta p
@@ -361,15 +349,19 @@ static int enqueue_iteration(MixRun& R) {
     const int V = (int)c->V;
     dim3 grid(nblocks(c->V), R.nrhs), blk(256);
     LMX_TRY(apply_ddagd_lo(R));
-    hipLaunchKernelGGL(k_mx_pap, grid, blk, 0, c->stream, R.w.p, R.w.Ap, V, R.w.partials, R.w.counters, R.w.rhs,
-                       R.w.batch);
+    const int nb = nblocks(c->V);
+    MixRhs* rhs = R.w.rhs;
+    MixBatch* bt = R.w.batch;
+    hipLaunchKernelGGL(k_mx_pap, grid, blk, 0, c->stream, R.w.p, R.w.Ap, V, R.w.partials, rhs);
+    hipLaunchKernelGGL(k_mx_fin<MX_PAP>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
     hipLaunchKernelGGL(k_mx_update_xr, grid, blk, 0, c->stream, R.w.xlo, R.w.rlo, R.w.p, R.w.Ap, V, R.w.partials,
-                       R.w.counters, R.w.rhs, R.w.batch);
-    hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, R.x, R.w.xlo, V, R.w.rhs, R.w.batch, 0);
-    LMX_TRY(apply_ddagd64(R, R.w.r64, R.x, R.w.t64, &R.w.batch->do_update));
-    hipLaunchKernelGGL(k_mx_true_res, grid, blk, 0, c->stream, R.b, R.w.r64, R.w.rlo, V, R.w.partials,
-                       R.w.counters, R.w.rhs, R.w.batch, 0);
-    hipLaunchKernelGGL(k_mx_update_p, grid, blk, 0, c->stream, R.w.rlo, R.w.p, V, R.w.rhs);
+                       rhs);
+    hipLaunchKernelGGL(k_mx_fin<MX_RR>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
+    hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, R.x, R.w.xlo, V, rhs, bt, 0);
+    LMX_TRY(apply_ddagd64(R, R.w.r64, R.x, R.w.t64, &bt->do_update));
+    hipLaunchKernelGGL(k_mx_true_res, grid, blk, 0, c->stream, R.b, R.w.r64, R.w.rlo, V, R.w.partials, rhs, bt, 0);
+    hipLaunchKernelGGL(k_mx_fin<MX_TRUE>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
+    hipLaunchKernelGGL(k_mx_update_p, grid, blk, 0, c->stream, R.w.rlo, R.w.p, V, rhs);
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }
@@ -425,12 +417,12 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     const int V = (int)c->V;
     dim3 grid(nblocks(c->V), nrhs), blk(256);
     const int rmax = params->reliable_maxiter > 0 ? params->reliable_maxiter : 100;
-    LMX_HIP(hipMemsetAsync(R.w.counters, 0, nrhs * sizeof(unsigned), c->stream));
     hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(64), 0, c->stream, R.w.rhs, R.w.batch, nrhs, params->reliable_delta,
                        rmax);
     LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
-    hipLaunchKernelGGL(k_mx_init, grid, blk, 0, c->stream, b, R.w.r64, R.w.rlo, R.w.p, R.w.xlo, V, params->rtol,
-                       params->atol, R.w.partials, R.w.counters, R.w.rhs, R.w.batch);
+    hipLaunchKernelGGL(k_mx_init, grid, blk, 0, c->stream, b, R.w.r64, R.w.rlo, R.w.p, R.w.xlo, V, R.w.partials);
+    hipLaunchKernelGGL(k_mx_fin<MX_INIT>, dim3(nrhs), blk, 0, c->stream, R.w.partials, nblocks(c->V), R.w.rhs,
+                       R.w.batch, params->rtol, params->atol, 0);
     LMX_HIP(hipGetLastError());
 
     const int chunk = params->check_every > 0 ? params->check_every : 10;
@@ -481,8 +473,10 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     // RHS stopped by maxiter: fold the inner solution in and report the true residual
     hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, x, R.w.xlo, V, R.w.rhs, R.w.batch, 1);
     LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
-    hipLaunchKernelGGL(k_mx_true_res, grid, blk, 0, c->stream, b, R.w.r64, R.w.rlo, V, R.w.partials, R.w.counters,
-                       R.w.rhs, R.w.batch, 1);
+    hipLaunchKernelGGL(k_mx_true_res, grid, blk, 0, c->stream, b, R.w.r64, R.w.rlo, V, R.w.partials, R.w.rhs,
+                       R.w.batch, 1);
+    hipLaunchKernelGGL(k_mx_fin<MX_TRUE>, dim3(nrhs), blk, 0, c->stream, R.w.partials, nblocks(c->V), R.w.rhs,
+                       R.w.batch, 0.0, 0.0, 1);
     LMX_HIP(hipGetLastError());
     LMX_HIP(hipMemcpyAsync(hs.data(), R.w.rhs, nrhs * sizeof(MixRhs), hipMemcpyDeviceToHost, c->stream));
     LMX_HIP(hipStreamSynchronize(c->stream));

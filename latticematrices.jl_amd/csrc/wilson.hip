@@ -79,13 +79,14 @@ __device__ __forceinline__ void clover_add(const Acc& F, int s, int V, const cx<
 // ---------------------------------------------------------------------------
 // One direction of the hopping sum for NR right-hand sides.
 // ---------------------------------------------------------------------------
-template <int MU, int FPM, int NR, class In, class Lk, typename R>
+template <int MU, int FPM, int NR, bool GP, class In, class Lk, typename R>
 __device__ __forceinline__ void hop_mu(cx<R> (&acc)[NR][12], const In& in, const Lk& U, const Geom& g, int s,
-                                       const int x[4], long in0, long rstride, int nr) {
+                                       const int x[4], const long (&roff)[NR]) {
     bool w;
     cx<R> u[9];
     // forward: U_mu(x) (1 + FPM gamma_mu) psi(x+mu)
     const int sp = nbr_fwd(s, x, MU, g, w);
+    const bool negf = w && g.phase_neg[MU];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         auto v = U.ld((long)(MU * 9 + k) * g.V + s);
@@ -93,18 +94,20 @@ __device__ __forceinline__ void hop_mu(cx<R> (&acc)[NR][12], const In& in, const
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        if (r < nr) {
-            cx<R> psi[12], h[6], o[6];
+        cx<R> psi[12], h[6], o[6];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) psi[k] = in.ld(in0 + r * rstride + (long)k * g.V + sp);
+        for (int k = 0; k < 12; ++k) psi[k] = in.ld(roff[r] + (long)k * g.V + sp);
+        if (GP) {
             if (w && !g.phase_one[MU]) phase_spinor(psi, g.ph_re[MU], g.ph_im[MU]);
-            half_project<FPM, MU>(psi, h);
-            matvec_fwd(u, h, o);
-            reconstruct_add<FPM, MU>(acc[r], o);
         }
+        half_project<FPM, MU>(psi, h);
+        if (!GP) neg_select(h, negf);
+        matvec_fwd(u, h, o);
+        reconstruct_add<FPM, MU>(acc[r], o);
     }
     // backward: U_mu(x-mu)^dagger (1 - FPM gamma_mu) psi(x-mu)
     const int sm = nbr_bwd(s, x, MU, g, w);
+    const bool negb = w && g.phase_neg[MU];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         auto v = U.ld((long)(MU * 9 + k) * g.V + sm);
@@ -112,15 +115,16 @@ __device__ __forceinline__ void hop_mu(cx<R> (&acc)[NR][12], const In& in, const
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        if (r < nr) {
-            cx<R> psi[12], h[6], o[6];
+        cx<R> psi[12], h[6], o[6];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) psi[k] = in.ld(in0 + r * rstride + (long)k * g.V + sm);
+        for (int k = 0; k < 12; ++k) psi[k] = in.ld(roff[r] + (long)k * g.V + sm);
+        if (GP) {
             if (w && !g.phase_one[MU]) phase_spinor(psi, g.iph_re[MU], g.iph_im[MU]);
-            half_project<-FPM, MU>(psi, h);
-            matvec_bwd(u, h, o);
-            reconstruct_add<-FPM, MU>(acc[r], o);
         }
+        half_project<-FPM, MU>(psi, h);
+        if (!GP) neg_select(h, negb);
+        matvec_bwd(u, h, o);
+        reconstruct_add<-FPM, MU>(acc[r], o);
     }
 }
 
@@ -136,14 +140,13 @@ enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
 enum { EPI_NONE = 0, EPI_PAP = 1 };
 
 // out = psi - kappa*hop (+ clover) or 1/2 hop, for NR right-hand sides per item.
-template <class In, class Lk, class Out, class Cl, int FPM, int MODE, int NR, int EPI>
+template <class In, class Lk, class Out, class Cl, int FPM, int MODE, int NR, int EPI, bool GP>
 __global__ void __launch_bounds__(256) k_wilson(Out out, In in, Lk U, Cl F, Geom g, double kappa_d, double ccoef_d,
                                                 long rstride, int ngroups, int nrhs, EpiArgs epi) {
     using R = typename In::R;
     if (epi.st != nullptr && epi.st->done) return;
     if (epi.gate != nullptr && *epi.gate == 0) return;
     __shared__ double lds4[4];
-    __shared__ int lds_flag;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int sb = lb / ngroups, rg = lb - sb * ngroups;
     const int s = sb * 256 + threadIdx.x;
@@ -155,14 +158,19 @@ __global__ void __launch_bounds__(256) k_wilson(Out out, In in, Lk U, Cl F, Geom
         int x[4];
         site_coords(s, g, x);
         cx<R> acc[NR][12];
+        // right-hand sides beyond nr (a ragged last group) recompute the
+        // group's first RHS and are not stored: no branch in the stencil
+        long roff[NR];
 #pragma unroll
-        for (int r = 0; r < NR; ++r)
+        for (int r = 0; r < NR; ++r) {
+            roff[r] = base + (long)(r < nr ? r : 0) * rstride;
 #pragma unroll
             for (int k = 0; k < 12; ++k) acc[r][k] = mk<R>(R(0), R(0));
-        hop_mu<0, FPM, NR>(acc, in, U, g, s, x, base, rstride, nr);
-        hop_mu<1, FPM, NR>(acc, in, U, g, s, x, base, rstride, nr);
-        hop_mu<2, FPM, NR>(acc, in, U, g, s, x, base, rstride, nr);
-        hop_mu<3, FPM, NR>(acc, in, U, g, s, x, base, rstride, nr);
+        }
+        hop_mu<0, FPM, NR, GP>(acc, in, U, g, s, x, roff);
+        hop_mu<1, FPM, NR, GP>(acc, in, U, g, s, x, roff);
+        hop_mu<2, FPM, NR, GP>(acc, in, U, g, s, x, roff);
+        hop_mu<3, FPM, NR, GP>(acc, in, U, g, s, x, roff);
         const R kappa = R(kappa_d);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
@@ -203,25 +211,9 @@ __global__ void __launch_bounds__(256) k_wilson(Out out, In in, Lk U, Cl F, Geom
         }
     }
     if constexpr (EPI == EPI_PAP) {
+        // block partial of Re <p, Ap>; k_finalize (FIN_PAP) completes the sum
         double tot = block_tree(red, lds4);
-        if (publish_partial(tot, epi.partials, lb, epi.counter, gridDim.x, &lds_flag)) {
-            double total = finish_partials(epi.partials, gridDim.x, lds4);
-            if (threadIdx.x == 0) {
-                CGState* st = epi.st;
-                // cg!: pAp = real(dot(p, Ap)); checks :107-114; alpha = rr/pAp :116
-                st->pAp = total;
-                if (!isfinite(total)) {
-                    st->done = 1;
-                    st->reason = LMX_CG_BREAKDOWN;
-                } else if (total <= 0.0) {
-                    st->done = 1;
-                    st->reason = LMX_CG_NONPOSITIVE_CURVATURE;
-                } else {
-                    st->alpha = st->rr / total;
-                }
-                *epi.counter = 0u;
-            }
-        }
+        if (threadIdx.x == 0) epi.partials[lb] = tot;
     }
 }
 
@@ -237,9 +229,15 @@ static int dispatch_fpm_mode(lmx_context* ctx, Out out, In in, Lk U, Cl F, doubl
     dim3 grid(nb), block(256);
     const bool dag = (flags & LMX_DAGGER) != 0;
     const bool donly = (flags & LMX_DONLY) != 0;
-#define LMX_LAUNCH(FPM, MODE)                                                                                  \
-    hipLaunchKernelGGL((k_wilson<In, Lk, Out, Cl, FPM, MODE, NR, EPI>), grid, block, 0, ctx->stream, out, in, U, F, \
-                       ctx->g, kappa, ccoef, rstride, ngroups, nrhs, epi)
+#define LMX_LAUNCH(FPM, MODE)                                                                                   \
+    do {                                                                                                         \
+        if (ctx->g.general)                                                                                      \
+            hipLaunchKernelGGL((k_wilson<In, Lk, Out, Cl, FPM, MODE, NR, EPI, true>), grid, block, 0, ctx->stream, \
+                               out, in, U, F, ctx->g, kappa, ccoef, rstride, ngroups, nrhs, epi);                \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_wilson<In, Lk, Out, Cl, FPM, MODE, NR, EPI, false>), grid, block, 0,           \
+                               ctx->stream, out, in, U, F, ctx->g, kappa, ccoef, rstride, ngroups, nrhs, epi);   \
+    } while (0)
     if (donly) {
         if constexpr (EPI == EPI_NONE) {
             if (dag)
@@ -285,14 +283,10 @@ int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover,
     F64In l{U};
     if (e.kind == 1) {
         LMX_CHECK(nrhs == 1, LMX_ERR_UNSUPPORTED, "fused p.Ap needs a single right-hand side");
-        if (clover) {
-            CloverWrap<F64Clover> F{F64Clover{clover}};
-            return dispatch_fpm_mode<F64In, F64In, F64Out, CloverWrap<F64Clover>, 1, EPI_PAP>(
-                ctx, o, i, l, F, kappa, ccoef, rstride, 1, 1, flags, true, epi);
-        }
-        CloverWrap<F64Clover> F{F64Clover{nullptr}};
-        return dispatch_fpm_mode<F64In, F64In, F64Out, CloverWrap<F64Clover>, 1, EPI_PAP>(
-            ctx, o, i, l, F, kappa, ccoef, rstride, 1, 1, flags, false, epi);
+        CloverWrap<F64Clover> F{F64Clover{clover}};
+        LMX_TRY((dispatch_fpm_mode<F64In, F64In, F64Out, CloverWrap<F64Clover>, 1, EPI_PAP>(
+            ctx, o, i, l, F, kappa, ccoef, rstride, 1, 1, flags, clover != nullptr, epi)));
+        return launch_finalize(ctx, nblocks(ctx->V), FIN_PAP, false, e.st, nullptr);
     }
     CloverWrap<F64Clover> F{F64Clover{clover}};
     return dispatch_fpm_mode<F64In, F64In, F64Out, CloverWrap<F64Clover>, 1, EPI_NONE>(
