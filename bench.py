@@ -4,8 +4,9 @@ Workload (N=1): 32^4 Wilson D^dagger D, weak-field SU(3) links
 U = exp(i 0.2 H) (seeded), Gaussian sources, kappa 0.124, antiperiodic in t,
 FP64 true relative residual 1e-10, zero start.
   - plain FP64 GPU CG (the reference algorithm cg!, one RHS), and
-  - the mixed-precision multi-RHS CG (FP16-stored links, FP32 spinors and
-    math, FP64 reliable updates / reductions / solution) on `--nrhs` sources.
+  - the mixed-precision multi-RHS CG (FP16 links, inner vectors as 16-bit
+    fixed point with a per-site scale, FP32 math; FP64 reliable updates /
+    reductions / solution) on `--nrhs` sources.
 One step = one full solve of the mixed multi-RHS batch; `value` is its
 time-to-solution per RHS.  Inputs are generated on the host (numpy, seeded)
 and uploaded before the timed region; the timed region holds device work
@@ -13,7 +14,8 @@ only (plus the solver's host convergence polls).
 
 Also measured: the FP64 D_W launch (roofline kernel, 960 B/site algorithmic:
 links 576 + psi 192 + out 192) timed with HIP events on the library stream,
-the mixed multi-RHS dslash (144/n + 192 B/site/RHS), and a plain 1 GiB copy
+the mixed multi-RHS dslash (links/n + in + out per site per RHS: 144/n + 104
+B with 16-bit spinors and FP16 links), and a plain 1 GiB copy
 kernel used to calibrate the rocprof byte counters.
 
 Prints one JSON line.  Run: python bench.py [--steps K] [--warmup W]
@@ -46,7 +48,7 @@ def parse():
     ap.add_argument("--eps", type=float, default=0.2)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--nrhs", type=int, default=12)
-    ap.add_argument("--precision", type=str, default="f16", choices=["f16", "f32"])
+    ap.add_argument("--precision", type=str, default="h16", choices=["h16", "f16", "f32"])
     ap.add_argument("--rhs-block", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--dslash-reps", type=int, default=50)
@@ -71,6 +73,14 @@ def timed(lat, fn, reps):
     e.record(lat.stream)
     e.synchronize()
     return s.elapsed_time(e) / reps
+
+
+def lo_dslash_bytes(precision, n):
+    """Algorithmic bytes per site per RHS of the low-precision dslash: the
+    links once per launch of n RHS, the input and output spinor once each."""
+    links = 288.0 if precision == "f32" else 144.0
+    spinor = 52.0 if precision == "h16" else 96.0
+    return links / n + 2 * spinor
 
 
 def copy_calibration(lat):
@@ -137,7 +147,7 @@ def main():
 
     if args.mode == "kernels":
         tune = lmx.autotune(D, args.precision, n, None)
-        bytes_dm = (144.0 if args.precision == "f16" else 288.0) / n + 192.0
+        bytes_dm = lo_dslash_bytes(args.precision, n)
         log(f"bench: {args.precision} dslash x{n}: {tune['ms_per_apply'] * 1e3:.1f} us/RHS, "
             f"{bytes_dm * V / (tune['ms_per_apply'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
         print(json.dumps({"mode": "kernels", "fp64_dslash_us": t_d64 * 1e3, "fp64_dslash_gbs": ach_d64,
@@ -162,7 +172,7 @@ def main():
 
     # ---- mixed multi-RHS CG ------------------------------------------------
     tune = lmx.autotune(D, args.precision, n, os.path.join(ROOT, "gpurun_out", "lmx_tune.txt"))
-    bytes_dm = (144.0 if args.precision == "f16" else 288.0) / n + 192.0  # B/site/RHS, links shared by n RHS
+    bytes_dm = lo_dslash_bytes(args.precision, n)
     ach_dm = bytes_dm * V / (tune["ms_per_apply"] * 1e-3) / 1e9
     log(f"bench: autotuned {args.precision} dslash x{n}: rhs_block {tune['rhs_block'] >> 4}/{tune['rhs_block'] & 15} "
         f"{tune['ms_per_apply'] * 1e3:.1f} us/RHS, {ach_dm:.0f} GB/s algorithmic")
@@ -207,7 +217,9 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.precision} links / f32 spinors+math, f64 reliable updates",
+        "dtype": {"h16": "16-bit fixed-point inner spinors, f16 links, f32 math, f64 reliable updates",
+                  "f16": "f16 links, f32 inner spinors and math, f64 reliable updates",
+                  "f32": "f32 inner links, spinors and math, f64 reliable updates"}[args.precision],
         "data": "synthetic (seeded weak-field SU(3) links eps=0.2, Gaussian sources)",
         "config": {"workload": f"Wilson D^dagger D CG {'x'.join(map(str, L))}, kappa {args.kappa}, rtol "
                                f"{args.rtol}, {n} RHS mixed vs 1 RHS FP64",
@@ -226,7 +238,7 @@ def main():
                          "frac": ach_dm / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "achieved": ach_d64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_d64 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_wilson FP64 D_W (32^4)", "bytes_per_launch": bytes_d64,
+                     "kernel": "k_stream FP64 D_W (32^4)", "bytes_per_launch": bytes_d64,
                      "us_per_launch": t_d64 * 1e3},
         "cpu_baseline": cpu,
     }

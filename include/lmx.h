@@ -49,7 +49,11 @@ enum {
 };
 
 /* ---- precisions of stored fields --------------------------------------- */
-enum { LMX_F64 = 0, LMX_F32 = 1, LMX_F16 = 2 };
+/* LMX_H16: spinors as 16-bit fixed point with one FP32 scale per site
+ * (value = q * scale, scale = largest |component| / 32767), links FP16; the
+ * mixed solver's inner vectors.  Spinor layout int4[nrhs][3][V] followed by
+ * float[nrhs][V]. */
+enum { LMX_F64 = 0, LMX_F32 = 1, LMX_F16 = 2, LMX_H16 = 3 };
 
 /* ---- operator flags ------------------------------------------------------ */
 enum {
@@ -74,7 +78,7 @@ typedef struct {
     double rtol;
     double atol;
     int32_t maxiter;
-    int32_t precision;        /* inner precision of lmx_cg_batched: LMX_F64 (plain), LMX_F32, LMX_F16 links */
+    int32_t precision;        /* inner precision of lmx_cg_batched: LMX_F64 (plain), LMX_F32, LMX_F16, LMX_H16 */
     double reliable_delta;    /* reliable update when |r_inner| < delta * |r at last update| */
     int32_t reliable_maxiter; /* ... or after this many inner iterations */
     int32_t check_every;      /* iterations between host convergence polls */
@@ -202,8 +206,9 @@ int lmx_cg_generic(lmx_context* ctx, lmx_apply_fn A, void* A_user, void* x, cons
 
 /* Batched D^dagger D solve of nrhs right-hand sides (x, b: nrhs stacked F64
  * fields, x is the initial guess).  params->precision selects plain FP64 CG
- * per RHS (LMX_F64) or the mixed solver: FP32 (LMX_F32) or FP16-stored links
- * with FP32 spinors (LMX_F16) inner CG with FP64 reliable updates, FP64
+ * per RHS (LMX_F64) or the mixed solver: FP32 (LMX_F32), FP16-stored links
+ * with FP32 spinors (LMX_F16), or FP16 links with 16-bit spinors (LMX_H16)
+ * inner CG (FP32 arithmetic) with FP64 reliable updates, FP64
  * reductions and FP64 solution accumulation.  Convergence is on the FP64
  * true residual |b - A x| <= max(atol, rtol |b|) per RHS.  The workspace is
  * caller-owned (size from lmx_cg_batched_workspace).  results[nrhs]. */

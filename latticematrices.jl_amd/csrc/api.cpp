@@ -6,6 +6,8 @@
 #include <mutex>
 #include <new>
 
+#include <cstdlib>
+
 #include "lmx_internal.h"
 
 namespace lmx {
@@ -167,6 +169,10 @@ int lmx_context_create(lmx_context** out, const int32_t dims[4], const double* p
     g.V = (int)V;
     c->V = V;
     c->stream = (hipStream_t)stream;
+    const char* tl = getenv("LMX_STREAM_TL");
+    const char* zs = getenv("LMX_STREAM_ZS");
+    c->tune_tl = tl ? atoi(tl) : 0;
+    c->tune_zs = zs ? atoi(zs) : 0;
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) {
         delete c;
@@ -225,12 +231,14 @@ int64_t lmx_context_volume(const lmx_context* c) { return c ? c->V : -1; }
 
 size_t lmx_spinor_bytes(const lmx_context* c, int precision, int nrhs) {
     if (!c) return 0;
-    size_t e = precision == LMX_F64 ? sizeof(double2) : precision == LMX_F32 ? sizeof(float2) : sizeof(__half2);
-    return e * 12 * (size_t)c->V * (size_t)std::max(nrhs, 0);
+    // LMX_H16: 12 16-bit (re, im) pairs plus one float scale per site
+    size_t e = precision == LMX_F64 ? 12 * sizeof(double2) : precision == LMX_F32 ? 12 * sizeof(float2)
+             : precision == LMX_H16 ? 12 * sizeof(int) + sizeof(float) : 12 * sizeof(__half2);
+    return e * (size_t)c->V * (size_t)std::max(nrhs, 0);
 }
 size_t lmx_links_bytes(const lmx_context* c, int precision) {
     if (!c) return 0;
-    size_t e = precision == LMX_F64 ? sizeof(double2) : precision == LMX_F32 ? sizeof(float2) : sizeof(__half2);
+    size_t e = precision == LMX_F64 ? sizeof(double2) : precision == LMX_F32 ? sizeof(float2) : sizeof(__half2);  // H16: FP16 links
     return e * 36 * (size_t)c->V;
 }
 size_t lmx_clover_bytes(const lmx_context* c, int precision) {

@@ -112,42 +112,217 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 
 // ---------------------------------------------------------------------------
-// Field accessors.  idx is a linear element index (component*V + site).
+// Field accessors.  A spinor accessor moves the 12 colour-spin components
+// (k = spin*3 + colour) of right-hand side r at site s; a link accessor the
+// 9 entries (row*3 + col) of U_mu at site s.  Layouts, per right-hand side
+// of V sites (components outermost, so a wave's load is one contiguous run):
+//   F64  double2[12][V]      F32  float2[12][V]
+//   H16  int4[3][V] + float[V]: components 4g..4g+3 as 16-bit fixed-point
+//        (re, im) pairs, value = q * scale[s], scale = max |component| / 32767
+// Indices are 32-bit within one right-hand side or link direction (12 V < 2^31).
 // ---------------------------------------------------------------------------
-struct F64In {
+// Output stores bypass the caches' normal allocation (nontemporal): the
+// stencil's output is not re-read by the same launch, and keeping it out of
+// L2 leaves room for the neighbour spinors and links that are.
+#ifndef LMX_NT_STORE
+#define LMX_NT_STORE 1
+#endif
+typedef double lmx_dv2 __attribute__((ext_vector_type(2)));
+typedef float lmx_fv2 __attribute__((ext_vector_type(2)));
+typedef int lmx_iv4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st_out(double2* p, double re, double im) {
+#if LMX_NT_STORE
+    lmx_dv2 v = {re, im};
+    __builtin_nontemporal_store(v, (lmx_dv2*)p);
+#else
+    *p = make_double2(re, im);
+#endif
+}
+__device__ __forceinline__ void st_out(float2* p, float re, float im) {
+#if LMX_NT_STORE
+    lmx_fv2 v = {re, im};
+    __builtin_nontemporal_store(v, (lmx_fv2*)p);
+#else
+    *p = make_float2(re, im);
+#endif
+}
+__device__ __forceinline__ void st_out(int4* p, int4 w) {
+#if LMX_NT_STORE
+    lmx_iv4 v = {w.x, w.y, w.z, w.w};
+    __builtin_nontemporal_store(v, (lmx_iv4*)p);
+#else
+    *p = w;
+#endif
+}
+__device__ __forceinline__ void st_out(float* p, float v) {
+#if LMX_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+struct SpF64In {
     using R = double;
     const double2* __restrict__ p;
-    __device__ __forceinline__ cx<double> ld(long i) const {
-        double2 v = p[i];
-        return mk<double>(v.x, v.y);
+    int V;
+    static SpF64In at(const void* q, const float*, int V, int r0) {
+        return SpF64In{(const double2*)q + (size_t)r0 * 12 * V, V};
+    }
+    __device__ __forceinline__ void ld12(int r, int s, cx<double>* z) const {
+        const double2* q = p + (size_t)r * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            double2 v = q[k * V + s];
+            z[k] = mk<double>(v.x, v.y);
+        }
     }
 };
-struct F32In {
-    using R = float;
-    const float2* __restrict__ p;
-    __device__ __forceinline__ cx<float> ld(long i) const {
-        float2 v = p[i];
-        return mk<float>(v.x, v.y);
-    }
-};
-struct F16In {
-    using R = float;
-    const __half2* __restrict__ p;
-    __device__ __forceinline__ cx<float> ld(long i) const {
-        float2 v = __half22float2(p[i]);
-        return mk<float>(v.x, v.y);
-    }
-};
-struct F64Out {
+struct SpF64Out {
     using R = double;
     double2* __restrict__ p;
-    __device__ __forceinline__ void st(long i, cx<double> z) const { p[i] = make_double2(z.re, z.im); }
+    int V;
+    __device__ __forceinline__ void st12(int r, int s, const cx<double>* z) const {
+        double2* q = p + (size_t)r * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) st_out(q + k * V + s, z[k].re, z[k].im);
+    }
 };
-struct F32Out {
+struct SpF32In {
+    using R = float;
+    const float2* __restrict__ p;
+    int V;
+    static SpF32In at(const void* q, const float*, int V, int r0) {
+        return SpF32In{(const float2*)q + (size_t)r0 * 12 * V, V};
+    }
+    __device__ __forceinline__ void ld12(int r, int s, cx<float>* z) const {
+        const float2* q = p + (size_t)r * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            float2 v = q[k * V + s];
+            z[k] = mk<float>(v.x, v.y);
+        }
+    }
+};
+struct SpF32Out {
     using R = float;
     float2* __restrict__ p;
-    __device__ __forceinline__ void st(long i, cx<float> z) const { p[i] = make_float2(z.re, z.im); }
+    int V;
+    static SpF32Out at(void* q, float*, int V, int r0) { return SpF32Out{(float2*)q + (size_t)r0 * 12 * V, V}; }
+    __device__ __forceinline__ void st12(int r, int s, const cx<float>* z) const {
+        float2* q = p + (size_t)r * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) st_out(q + k * V + s, z[k].re, z[k].im);
+    }
 };
+
+constexpr float kH16Max = 32767.0f;
+
+__device__ __forceinline__ float h16_lo(int w) { return float(int(unsigned(w) << 16) >> 16); }
+__device__ __forceinline__ float h16_hi(int w) { return float(w >> 16); }
+__device__ __forceinline__ int h16_pack(float re, float im) {
+    const unsigned a = unsigned(__float2int_rn(re)), b = unsigned(__float2int_rn(im));
+    return int((a & 0xffffu) | (b << 16));
+}
+
+struct SpH16In {
+    using R = float;
+    const int4* __restrict__ p;
+    const float* __restrict__ scale;
+    int V;
+    static SpH16In at(const void* q, const float* n, int V, int r0) {
+        return SpH16In{(const int4*)q + (size_t)r0 * 3 * V, n + (size_t)r0 * V, V};
+    }
+    __device__ __forceinline__ void ld12(int r, int s, cx<float>* z) const {
+        const int4* q = p + (size_t)r * 3 * V;
+        const float c = scale[(size_t)r * V + s];
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            int4 v = q[g * V + s];
+            z[4 * g + 0] = mk<float>(h16_lo(v.x) * c, h16_hi(v.x) * c);
+            z[4 * g + 1] = mk<float>(h16_lo(v.y) * c, h16_hi(v.y) * c);
+            z[4 * g + 2] = mk<float>(h16_lo(v.z) * c, h16_hi(v.z) * c);
+            z[4 * g + 3] = mk<float>(h16_lo(v.w) * c, h16_hi(v.w) * c);
+        }
+    }
+};
+struct SpH16Out {
+    using R = float;
+    int4* __restrict__ p;
+    float* __restrict__ scale;
+    int V;
+    static SpH16Out at(void* q, float* n, int V, int r0) {
+        return SpH16Out{(int4*)q + (size_t)r0 * 3 * V, n + (size_t)r0 * V, V};
+    }
+    __device__ __forceinline__ void st12(int r, int s, const cx<float>* z) const {
+        float m = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(z[k].re), fabsf(z[k].im)));
+        const float inv = m > 0.0f ? kH16Max / m : 0.0f;
+        int4* q = p + (size_t)r * 3 * V;
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            int4 v;
+            v.x = h16_pack(z[4 * g + 0].re * inv, z[4 * g + 0].im * inv);
+            v.y = h16_pack(z[4 * g + 1].re * inv, z[4 * g + 1].im * inv);
+            v.z = h16_pack(z[4 * g + 2].re * inv, z[4 * g + 2].im * inv);
+            v.w = h16_pack(z[4 * g + 3].re * inv, z[4 * g + 3].im * inv);
+            st_out(q + g * V + s, v);
+        }
+        st_out(scale + (size_t)r * V + s, m / kH16Max);
+    }
+};
+
+struct LinkF64 {
+    const double2* __restrict__ p;
+    int V;
+    template <typename R>
+    __device__ __forceinline__ void ld9(int mu, int s, cx<R>* u) const {
+        const double2* q = p + (size_t)mu * 9 * V;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            double2 v = q[k * V + s];
+            u[k] = mk<R>(R(v.x), R(v.y));
+        }
+    }
+};
+struct LinkF32 {
+    const float2* __restrict__ p;
+    int V;
+    template <typename R>
+    __device__ __forceinline__ void ld9(int mu, int s, cx<R>* u) const {
+        const float2* q = p + (size_t)mu * 9 * V;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            float2 v = q[k * V + s];
+            u[k] = mk<R>(R(v.x), R(v.y));
+        }
+    }
+};
+struct LinkF16 {
+    const __half2* __restrict__ p;
+    int V;
+    template <typename R>
+    __device__ __forceinline__ void ld9(int mu, int s, cx<R>* u) const {
+        const __half2* q = p + (size_t)mu * 9 * V;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            float2 v = __half22float2(q[k * V + s]);
+            u[k] = mk<R>(R(v.x), R(v.y));
+        }
+    }
+};
+
+// Keep a 12-component accumulator in registers at this point: an empty asm
+// that "modifies" the values stops the scheduler from sinking the arithmetic
+// that produced them below the loads that follow (which otherwise keeps
+// every loaded neighbour live at once and spills).
+template <typename R>
+__device__ __forceinline__ void pin_acc(cx<R> (&a)[12]) {
+#pragma unroll
+    for (int k = 0; k < 12; k += 2)
+        asm volatile("" : "+v"(a[k].re), "+v"(a[k].im), "+v"(a[k + 1].re), "+v"(a[k + 1].im)::"memory");
+}
 
 // ---------------------------------------------------------------------------
 // Spin projection / reconstruction (Appendix of the half-spin kernels)

@@ -65,6 +65,9 @@ struct lmx_context {
     lmx::CGState* host_state;  // pinned mirror
     double* dot_out;      // device scalars for lmx_dot
     double* host_dot;     // pinned mirror
+    // Stencil launch-shape overrides (0 = automatic): t-slices per workgroup
+    // and 3-D sub-ranges, from LMX_STREAM_TL / LMX_STREAM_ZS at creation.
+    int tune_tl, tune_zs;
 };
 
 struct lmx_wilson {
@@ -80,7 +83,7 @@ struct lmx_wilson {
     __half2* clover16;
     bool lo_valid;
     // Autotuned launch shape per (precision, nrhs); 0 = untuned.
-    int tuned_block[3][65];
+    int tuned_block[4][65];
 };
 
 namespace lmx {
@@ -103,8 +106,27 @@ struct Epilogue {
 int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
                       double2* out, const double2* in, int nrhs, int flags, const Epilogue& epi);
 
-int launch_wilson_lo(lmx_context* ctx, int link_prec, const void* U, const void* clover, double kappa, double csw,
-                     float2* out, const float2* in, int nrhs, int flags, int rhs_block, const int* gate);
+// Low-precision apply: spinors LMX_F32 or LMX_H16 (data + per-site scale),
+// links LMX_F32 or LMX_F16, FP32 arithmetic.  With partials set, the launch
+// also stores block partials of Re <aux, out> per RHS (row r at
+// partials + r * pstride, one per 256 consecutive sites).
+struct LoArgs {
+    int link_prec, spinor_prec;
+    const void* links;
+    const void* clover;
+    double kappa, csw;
+    void* out;
+    float* out_norm;
+    const void* in;
+    const float* in_norm;
+    const void* aux;
+    const float* aux_norm;
+    double* partials;
+    long pstride;
+    int nrhs, flags, rhs_block;
+    const int* gate;
+};
+int launch_wilson_lo(lmx_context* ctx, const LoArgs& a);
 
 int launch_clover_build_f64(lmx_context* ctx, const double2* U, double2* clover);
 int launch_clover_export(lmx_context* ctx, const double2* clover, double2* F_out);

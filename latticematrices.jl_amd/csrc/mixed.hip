@@ -36,12 +36,16 @@ struct MixBatch {
     int pad;
 };
 
+// An inner (low-precision) vector of nrhs fields: FP32 (n == null) or
+// 16-bit fixed point with per-site scales n.
+struct InnerVec {
+    void* p;
+    float* n;
+};
+
 struct MixPtrs {
     float2* xlo;
-    float2* rlo;
-    float2* p;
-    float2* Ap;
-    float2* t32;
+    InnerVec r, p, Ap, t;
     double2* r64;
     double2* t64;
     double* partials;
@@ -51,7 +55,7 @@ struct MixPtrs {
 
 static inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-static size_t layout(lmx_context* ctx, int nrhs, MixPtrs* P, char* base) {
+static size_t layout(lmx_context* ctx, int nrhs, int sp, MixPtrs* P, char* base) {
     const size_t V = (size_t)ctx->V;
     const size_t f32 = 12 * V * sizeof(float2) * nrhs, f64 = 12 * V * sizeof(double2) * nrhs;
     const size_t nb = (size_t)nblocks(ctx->V);
@@ -61,12 +65,23 @@ static size_t layout(lmx_context* ctx, int nrhs, MixPtrs* P, char* base) {
         off += align256(bytes);
         return p;
     };
+    auto inner = [&]() {
+        InnerVec v;
+        if (sp == LMX_H16) {
+            v.p = take(3 * V * sizeof(int4) * nrhs);
+            v.n = (float*)take(V * sizeof(float) * nrhs);
+        } else {
+            v.p = take(f32);
+            v.n = nullptr;
+        }
+        return v;
+    };
     MixPtrs q;
     q.xlo = (float2*)take(f32);
-    q.rlo = (float2*)take(f32);
-    q.p = (float2*)take(f32);
-    q.Ap = (float2*)take(f32);
-    q.t32 = (float2*)take(f32);
+    q.r = inner();
+    q.p = inner();
+    q.Ap = inner();
+    q.t = inner();
     q.r64 = (double2*)take(f64);
     q.t64 = (double2*)take(f64);
     q.partials = (double*)take(2 * nb * nrhs * sizeof(double));
@@ -165,29 +180,30 @@ __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ parti
     }
 }
 
-// r64 = b - A x (A x already in r64); r_lo = r64; p = r_lo; x_lo = 0;
-// |r|^2 and |b|^2 per RHS.
-__global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, double2* __restrict__ r64,
-                                                 float2* __restrict__ rlo, float2* __restrict__ p,
-                                                 float2* __restrict__ xlo, int V, double* partials) {
+// r64 = b - A x (A x already in r64); r_lo = p = r64 (rounded to the inner
+// format); x_lo = 0; |r|^2 and |b|^2 per RHS.
+template <class SO>
+__global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, double2* __restrict__ r64, SO rlo,
+                                                 SO p, float2* __restrict__ xlo, int V, double* partials) {
     const int i = blockIdx.y;
     const int s = blockIdx.x * 256 + threadIdx.x;
     double rr = 0.0, bb = 0.0;
     if (s < V) {
-        const long o = (long)i * 12 * V;
+        const size_t o = (size_t)i * 12 * V;
+        cx<float> f[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            const long e = o + (long)k * V + s;
+            const size_t e = o + (size_t)k * V + s;
             double2 bv = b[e], av = r64[e];
             double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
             r64[e] = rv;
-            float2 f = make_float2((float)rv.x, (float)rv.y);
-            rlo[e] = f;
-            p[e] = f;
+            f[k] = mk<float>((float)rv.x, (float)rv.y);
             xlo[e] = make_float2(0.f, 0.f);
             rr += rv.x * rv.x + rv.y * rv.y;
             bb += bv.x * bv.x + bv.y * bv.y;
         }
+        rlo.st12(i, s, f);
+        p.st12(i, s, f);
     }
     row_partials(rr, bb, partials);
 }
@@ -208,47 +224,33 @@ __global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta,
     }
 }
 
-// pAp_i = Re <p_i, Ap_i> (FP64 accumulation)
-__global__ void __launch_bounds__(256) k_mx_pap(const float2* __restrict__ p, const float2* __restrict__ Ap, int V,
-                                                double* partials, const MixRhs* rhs) {
-    const int i = blockIdx.y;
-    if (rhs[i].done) return;
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    double v = 0.0;
-    if (s < V) {
-        const long o = (long)i * 12 * V;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            float2 a = p[o + (long)k * V + s], c = Ap[o + (long)k * V + s];
-            v += (double)a.x * (double)c.x + (double)a.y * (double)c.y;
-        }
-    }
-    row_partials(v, 0.0, partials);
-}
-
-// x_lo += alpha p; r_lo -= alpha Ap; |r_lo|^2
-__global__ void __launch_bounds__(256) k_mx_update_xr(float2* __restrict__ xlo, float2* __restrict__ rlo,
-                                                      const float2* __restrict__ p, const float2* __restrict__ Ap,
-                                                      int V, double* partials, const MixRhs* rhs) {
+// x_lo += alpha p; r_lo -= alpha Ap; |r_lo|^2 (of the unrounded update)
+template <class SI, class SO>
+__global__ void __launch_bounds__(256) k_mx_update_xr(float2* __restrict__ xlo, SO rout, SI rin, SI p, SI Ap, int V,
+                                                      double* partials, const MixRhs* rhs) {
     const int i = blockIdx.y;
     if (rhs[i].done) return;
     const float a = (float)rhs[i].alpha;
     const int s = blockIdx.x * 256 + threadIdx.x;
     double v = 0.0;
     if (s < V) {
-        const long o = (long)i * 12 * V;
+        const size_t o = (size_t)i * 12 * V;
+        cx<float> pv[12], av[12], rv[12];
+        p.ld12(i, s, pv);
+        Ap.ld12(i, s, av);
+        rin.ld12(i, s, rv);
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            const long e = o + (long)k * V + s;
-            float2 pv = p[e], av = Ap[e], xv = xlo[e], rv = rlo[e];
-            xv.x = fmaf(a, pv.x, xv.x);
-            xv.y = fmaf(a, pv.y, xv.y);
-            rv.x = fmaf(-a, av.x, rv.x);
-            rv.y = fmaf(-a, av.y, rv.y);
+            const size_t e = o + (size_t)k * V + s;
+            float2 xv = xlo[e];
+            xv.x = fmaf(a, pv[k].re, xv.x);
+            xv.y = fmaf(a, pv[k].im, xv.y);
             xlo[e] = xv;
-            rlo[e] = rv;
-            v += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+            rv[k].re = fmaf(-a, av[k].re, rv[k].re);
+            rv[k].im = fmaf(-a, av[k].im, rv[k].im);
+            v += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
         }
+        rout.st12(i, s, rv);
     }
     row_partials(v, 0.0, partials);
 }
@@ -260,10 +262,10 @@ __global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float
     if ((!force && !batch->do_update) || rhs[i].done) return;
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= V) return;
-    const long o = (long)i * 12 * V;
+    const size_t o = (size_t)i * 12 * V;
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-        const long e = o + (long)k * V + s;
+        const size_t e = o + (size_t)k * V + s;
         float2 l = xlo[e];
         double2 h = x[e];
         x[e] = make_double2(h.x + (double)l.x, h.y + (double)l.y);
@@ -272,43 +274,45 @@ __global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float
 }
 
 // reliable update, part 2: r64 = b - A x (A x in r64), r_lo = r64, |r|^2 in FP64
-__global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__ b, double2* __restrict__ r64,
-                                                     float2* __restrict__ rlo, int V, double* partials,
-                                                     const MixRhs* rhs, const MixBatch* batch, int force) {
+template <class SO>
+__global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__ b, double2* __restrict__ r64, SO rlo,
+                                                     int V, double* partials, const MixRhs* rhs,
+                                                     const MixBatch* batch, int force) {
     const int i = blockIdx.y;
     if ((!force && !batch->do_update) || rhs[i].done) return;
     const int s = blockIdx.x * 256 + threadIdx.x;
     double v = 0.0;
     if (s < V) {
-        const long o = (long)i * 12 * V;
+        const size_t o = (size_t)i * 12 * V;
+        cx<float> f[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            const long e = o + (long)k * V + s;
+            const size_t e = o + (size_t)k * V + s;
             double2 bv = b[e], av = r64[e];
             double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
             r64[e] = rv;
-            rlo[e] = make_float2((float)rv.x, (float)rv.y);
+            f[k] = mk<float>((float)rv.x, (float)rv.y);
             v += rv.x * rv.x + rv.y * rv.y;
         }
+        rlo.st12(i, s, f);
     }
     row_partials(v, 0.0, partials);
 }
 
 // p = r_lo + beta p
-__global__ void __launch_bounds__(256) k_mx_update_p(const float2* __restrict__ rlo, float2* __restrict__ p, int V,
-                                                     const MixRhs* rhs) {
+template <class SI, class SO>
+__global__ void __launch_bounds__(256) k_mx_update_p(SI r, SI pin, SO pout, int V, const MixRhs* rhs) {
     const int i = blockIdx.y;
     if (rhs[i].done) return;
     const float be = (float)rhs[i].beta;
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= V) return;
-    const long o = (long)i * 12 * V;
+    cx<float> rv[12], pv[12];
+    r.ld12(i, s, rv);
+    pin.ld12(i, s, pv);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-        const long e = o + (long)k * V + s;
-        float2 r = rlo[e], q = p[e];
-        p[e] = make_float2(fmaf(be, q.x, r.x), fmaf(be, q.y, r.y));
-    }
+    for (int k = 0; k < 12; ++k) pv[k] = mk<float>(fmaf(be, pv[k].re, rv[k].re), fmaf(be, pv[k].im, rv[k].im));
+    pout.st12(i, s, pv);
 }
 
 // ---------------------------------------------------------------------------
@@ -322,8 +326,18 @@ struct MixRun {
     const double2* b;
     int nrhs;
     int link_prec;
+    int spinor_prec;
     int rhs_block;
 };
+
+template <class SI>
+static SI in_of(const InnerVec& v, int V) {
+    return SI::at(v.p, v.n, V, 0);
+}
+template <class SO>
+static SO out_of(const InnerVec& v, int V) {
+    return SO::at(v.p, v.n, V, 0);
+}
 
 static int apply_ddagd64(MixRun& R, double2* out, const double2* in, double2* tmp, const int* gate) {
     Epilogue e{0, nullptr, nullptr, nullptr, nullptr, gate};
@@ -332,50 +346,110 @@ static int apply_ddagd64(MixRun& R, double2* out, const double2* in, double2* tm
     return launch_wilson_f64(R.ctx, op->links, op->clover64, op->kappa, op->csw, out, tmp, R.nrhs, LMX_DAGGER, e);
 }
 
+static LoArgs lo_args(lmx_wilson* op, int link_prec, int spinor_prec) {
+    LoArgs a;
+    memset(&a, 0, sizeof(a));
+    a.link_prec = link_prec;
+    a.spinor_prec = spinor_prec;
+    a.links = link_prec == LMX_F16 ? (const void*)op->links16 : (const void*)op->links32;
+    a.clover = op->clover64 ? (link_prec == LMX_F16 ? (const void*)op->clover16 : (const void*)op->clover32) : nullptr;
+    a.kappa = op->kappa;
+    a.csw = op->csw;
+    return a;
+}
+
+// t = D p; Ap = D^dagger t with the block partials of Re <p, Ap> fused in.
 static int apply_ddagd_lo(MixRun& R) {
-    lmx_wilson* op = R.op;
-    const void* U = R.link_prec == LMX_F16 ? (const void*)op->links16 : (const void*)op->links32;
-    const void* F = op->clover64 ? (R.link_prec == LMX_F16 ? (const void*)op->clover16 : (const void*)op->clover32)
-                                 : nullptr;
-    const int* gate = &R.w.batch->n_active;
-    LMX_TRY(launch_wilson_lo(R.ctx, R.link_prec, U, F, op->kappa, op->csw, R.w.t32, R.w.p, R.nrhs, 0, R.rhs_block,
-                             gate));
-    return launch_wilson_lo(R.ctx, R.link_prec, U, F, op->kappa, op->csw, R.w.Ap, R.w.t32, R.nrhs, LMX_DAGGER,
-                            R.rhs_block, gate);
+    LoArgs a = lo_args(R.op, R.link_prec, R.spinor_prec);
+    a.nrhs = R.nrhs;
+    a.rhs_block = R.rhs_block;
+    a.gate = &R.w.batch->n_active;
+    a.out = R.w.t.p;
+    a.out_norm = R.w.t.n;
+    a.in = R.w.p.p;
+    a.in_norm = R.w.p.n;
+    a.flags = 0;
+    LMX_TRY(launch_wilson_lo(R.ctx, a));
+    a.out = R.w.Ap.p;
+    a.out_norm = R.w.Ap.n;
+    a.in = R.w.t.p;
+    a.in_norm = R.w.t.n;
+    a.aux = R.w.p.p;
+    a.aux_norm = R.w.p.n;
+    a.partials = R.w.partials;
+    a.pstride = 2L * nblocks(R.ctx->V);
+    a.flags = LMX_DAGGER;
+    return launch_wilson_lo(R.ctx, a);
+}
+
+template <class SI, class SO>
+static void enqueue_blas_xr(MixRun& R, dim3 grid) {
+    const int V = (int)R.ctx->V;
+    hipLaunchKernelGGL((k_mx_update_xr<SI, SO>), grid, dim3(256), 0, R.ctx->stream, R.w.xlo, out_of<SO>(R.w.r, V),
+                       in_of<SI>(R.w.r, V), in_of<SI>(R.w.p, V), in_of<SI>(R.w.Ap, V), V, R.w.partials, R.w.rhs);
+}
+template <class SO>
+static void enqueue_true_res(MixRun& R, dim3 grid, int force) {
+    const int V = (int)R.ctx->V;
+    hipLaunchKernelGGL((k_mx_true_res<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V), V,
+                       R.w.partials, R.w.rhs, R.w.batch, force);
+}
+template <class SI, class SO>
+static void enqueue_update_p(MixRun& R, dim3 grid) {
+    const int V = (int)R.ctx->V;
+    hipLaunchKernelGGL((k_mx_update_p<SI, SO>), grid, dim3(256), 0, R.ctx->stream, in_of<SI>(R.w.r, V),
+                       in_of<SI>(R.w.p, V), out_of<SO>(R.w.p, V), V, R.w.rhs);
+}
+template <class SO>
+static void enqueue_init(MixRun& R, dim3 grid) {
+    const int V = (int)R.ctx->V;
+    hipLaunchKernelGGL((k_mx_init<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V),
+                       out_of<SO>(R.w.p, V), R.w.xlo, V, R.w.partials);
 }
 
 static int enqueue_iteration(MixRun& R) {
     lmx_context* c = R.ctx;
     const int V = (int)c->V;
+    const bool h16 = R.spinor_prec == LMX_H16;
     dim3 grid(nblocks(c->V), R.nrhs), blk(256);
     LMX_TRY(apply_ddagd_lo(R));
     const int nb = nblocks(c->V);
     MixRhs* rhs = R.w.rhs;
     MixBatch* bt = R.w.batch;
-    hipLaunchKernelGGL(k_mx_pap, grid, blk, 0, c->stream, R.w.p, R.w.Ap, V, R.w.partials, rhs);
     hipLaunchKernelGGL(k_mx_fin<MX_PAP>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
-    hipLaunchKernelGGL(k_mx_update_xr, grid, blk, 0, c->stream, R.w.xlo, R.w.rlo, R.w.p, R.w.Ap, V, R.w.partials,
-                       rhs);
+    if (h16)
+        enqueue_blas_xr<SpH16In, SpH16Out>(R, grid);
+    else
+        enqueue_blas_xr<SpF32In, SpF32Out>(R, grid);
     hipLaunchKernelGGL(k_mx_fin<MX_RR>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
     hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, R.x, R.w.xlo, V, rhs, bt, 0);
     LMX_TRY(apply_ddagd64(R, R.w.r64, R.x, R.w.t64, &bt->do_update));
-    hipLaunchKernelGGL(k_mx_true_res, grid, blk, 0, c->stream, R.b, R.w.r64, R.w.rlo, V, R.w.partials, rhs, bt, 0);
+    if (h16)
+        enqueue_true_res<SpH16Out>(R, grid, 0);
+    else
+        enqueue_true_res<SpF32Out>(R, grid, 0);
     hipLaunchKernelGGL(k_mx_fin<MX_TRUE>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
-    hipLaunchKernelGGL(k_mx_update_p, grid, blk, 0, c->stream, R.w.rlo, R.w.p, V, rhs);
+    if (h16)
+        enqueue_update_p<SpH16In, SpH16Out>(R, grid);
+    else
+        enqueue_update_p<SpF32In, SpF32Out>(R, grid);
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }
 
+static int spinor_prec_of(int precision) { return precision == LMX_H16 ? LMX_H16 : LMX_F32; }
+static int link_prec_of(int precision) { return precision == LMX_F32 ? LMX_F32 : LMX_F16; }
+
 size_t cg_batched_workspace_impl(lmx_wilson* op, int nrhs, const lmx_cg_params* params) {
     if (params->precision == LMX_F64) return align256(4 * 12 * (size_t)op->ctx->V * sizeof(double2));
-    return layout(op->ctx, std::max(nrhs, 1), nullptr, nullptr);
+    return layout(op->ctx, std::max(nrhs, 1), spinor_prec_of(params->precision), nullptr, nullptr);
 }
 
 int lmx_cg_ddagd_internal(lmx_wilson* op, double2* x, const double2* b, double2* r, double2* p, double2* Ap,
                           double2* tmp, const lmx_cg_params* params, lmx_cg_result* result);
 
 static int default_rhs_block(lmx_wilson* op, int prec, int nrhs) {
-    const int t = (prec >= 0 && prec < 3 && nrhs >= 0 && nrhs <= 64) ? op->tuned_block[prec][nrhs] : 0;
+    const int t = (prec >= 0 && prec < 4 && nrhs >= 0 && nrhs <= 64) ? op->tuned_block[prec][nrhs] : 0;
     if (t > 0) return t;
     // all RHS in one launch, two per work item when that divides evenly
     const int per = std::min(nrhs, 12);
@@ -399,8 +473,8 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
                                           results + i));
         return LMX_OK;
     }
-    LMX_CHECK(params->precision == LMX_F32 || params->precision == LMX_F16, LMX_ERR_ARG, "unknown precision %d",
-              params->precision);
+    LMX_CHECK(params->precision == LMX_F32 || params->precision == LMX_F16 || params->precision == LMX_H16,
+              LMX_ERR_ARG, "unknown precision %d", params->precision);
     LMX_CHECK(nrhs <= kMaxStates, LMX_ERR_ARG, "at most %d right-hand sides per batch", kMaxStates);
     LMX_CHECK(params->reliable_delta > 0.0 && params->reliable_delta < 1.0, LMX_ERR_ARG,
               "reliable_delta must be in (0, 1)");
@@ -408,11 +482,12 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     MixRun R;
     R.op = op;
     R.ctx = c;
-    layout(c, nrhs, &R.w, (char*)work);
+    R.spinor_prec = spinor_prec_of(params->precision);
+    R.link_prec = link_prec_of(params->precision);
+    layout(c, nrhs, R.spinor_prec, &R.w, (char*)work);
     R.x = x;
     R.b = b;
     R.nrhs = nrhs;
-    R.link_prec = params->precision == LMX_F16 ? LMX_F16 : LMX_F32;
     R.rhs_block = params->rhs_block > 0 ? params->rhs_block : default_rhs_block(op, params->precision, nrhs);
     const int V = (int)c->V;
     dim3 grid(nblocks(c->V), nrhs), blk(256);
@@ -420,7 +495,10 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(64), 0, c->stream, R.w.rhs, R.w.batch, nrhs, params->reliable_delta,
                        rmax);
     LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
-    hipLaunchKernelGGL(k_mx_init, grid, blk, 0, c->stream, b, R.w.r64, R.w.rlo, R.w.p, R.w.xlo, V, R.w.partials);
+    if (R.spinor_prec == LMX_H16)
+        enqueue_init<SpH16Out>(R, grid);
+    else
+        enqueue_init<SpF32Out>(R, grid);
     hipLaunchKernelGGL(k_mx_fin<MX_INIT>, dim3(nrhs), blk, 0, c->stream, R.w.partials, nblocks(c->V), R.w.rhs,
                        R.w.batch, params->rtol, params->atol, 0);
     LMX_HIP(hipGetLastError());
@@ -473,8 +551,10 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     // RHS stopped by maxiter: fold the inner solution in and report the true residual
     hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, x, R.w.xlo, V, R.w.rhs, R.w.batch, 1);
     LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
-    hipLaunchKernelGGL(k_mx_true_res, grid, blk, 0, c->stream, b, R.w.r64, R.w.rlo, V, R.w.partials, R.w.rhs,
-                       R.w.batch, 1);
+    if (R.spinor_prec == LMX_H16)
+        enqueue_true_res<SpH16Out>(R, grid, 1);
+    else
+        enqueue_true_res<SpF32Out>(R, grid, 1);
     hipLaunchKernelGGL(k_mx_fin<MX_TRUE>, dim3(nrhs), blk, 0, c->stream, R.w.partials, nblocks(c->V), R.w.rhs,
                        R.w.batch, 0.0, 0.0, 1);
     LMX_HIP(hipGetLastError());
@@ -496,7 +576,7 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
 
 // ---------------------------------------------------------------------------
 // Autotuner: time the inner (low-precision) dslash for every RHS blocking
-// (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,3,4}) and keep
+// (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,4}) and keep
 // the fastest per (volume, precision, nrhs, arch); optionally persisted as
 // text lines "V precision nrhs arch rhs_block us_per_rhs".
 // ---------------------------------------------------------------------------
@@ -534,7 +614,8 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     using namespace lmx;
     clear_error();
     LMX_CHECK(op != nullptr && op->ctx != nullptr && out != nullptr, LMX_ERR_ARG, "bad autotune arguments");
-    LMX_CHECK(precision == LMX_F32 || precision == LMX_F16, LMX_ERR_ARG, "autotune precision must be F32 or F16");
+    LMX_CHECK(precision == LMX_F32 || precision == LMX_F16 || precision == LMX_H16, LMX_ERR_ARG,
+              "autotune precision must be F32, F16 or H16");
     LMX_CHECK(nrhs >= 1 && nrhs <= 64, LMX_ERR_ARG, "nrhs must be in [1, 64]");
     lmx_context* c = op->ctx;
     const std::string arch = arch_name(c->device);
@@ -548,8 +629,9 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
         return LMX_OK;
     }
     LMX_TRY(ensure_lo_links(op));
-    const size_t fb = 12 * (size_t)c->V * nrhs * sizeof(float2);
-    float2 *in = nullptr, *o = nullptr;
+    const int sp = spinor_prec_of(precision);
+    const size_t fb = lmx_spinor_bytes(c, sp, nrhs);
+    char *in = nullptr, *o = nullptr;
     LMX_HIP(hipMalloc(&in, fb));
     if (hipMalloc(&o, fb) != hipSuccess) {
         (void)hipFree(in);
@@ -560,9 +642,14 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    const void* U = precision == LMX_F16 ? (const void*)op->links16 : (const void*)op->links32;
-    const void* F = op->clover64 ? (precision == LMX_F16 ? (const void*)op->clover16 : (const void*)op->clover32)
-                                 : nullptr;
+    LoArgs la = lo_args(op, link_prec_of(precision), sp);
+    la.nrhs = nrhs;
+    la.in = in;
+    la.out = o;
+    if (sp == LMX_H16) {  // per-site scales follow the 16-bit data
+        la.in_norm = (const float*)(in + (size_t)nrhs * c->V * 3 * sizeof(int4));
+        la.out_norm = (float*)(o + (size_t)nrhs * c->V * 3 * sizeof(int4));
+    }
     const int per_opts[5] = {1, 2, 4, 6, 12};
     int rc = LMX_OK;
     best = 0;
@@ -571,14 +658,13 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
         const int per = per_opts[pi];
         if (per > nrhs) break;
         for (int nr = 1; nr <= 4 && rc == LMX_OK; ++nr) {
-            if (nr > per || per % nr != 0) continue;
+            if (nr == 3 || nr > per || per % nr != 0) continue;  // built: 1, 2, 4 per work item
             const int blk = (per << 4) | nr;
-            for (int k = 0; k < 2 && rc == LMX_OK; ++k)
-                rc = launch_wilson_lo(c, precision, U, F, op->kappa, op->csw, o, in, nrhs, 0, blk, nullptr);
+            la.rhs_block = blk;
+            for (int k = 0; k < 2 && rc == LMX_OK; ++k) rc = launch_wilson_lo(c, la);
             const int reps = 5;
             (void)hipEventRecord(e0, c->stream);
-            for (int k = 0; k < reps && rc == LMX_OK; ++k)
-                rc = launch_wilson_lo(c, precision, U, F, op->kappa, op->csw, o, in, nrhs, 0, blk, nullptr);
+            for (int k = 0; k < reps && rc == LMX_OK; ++k) rc = launch_wilson_lo(c, la);
             (void)hipEventRecord(e1, c->stream);
             if (hipEventSynchronize(e1) != hipSuccess) rc = LMX_ERR_HIP;
             float ms = 0.f;

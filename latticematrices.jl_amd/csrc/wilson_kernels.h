@@ -1,0 +1,506 @@
+// wilson_kernels.h -- the Wilson(-clover) hopping stencil on gfx950 (kernel
+// templates; wilson.hip instantiates FP64, wilson_lo.hip the low precisions).
+//
+// Two launch shapes share the per-direction arithmetic:
+//  * k_stream: a workgroup owns 256 consecutive sites of one t-slice (a 3-D
+//    block) and walks TL consecutive t-slices.  The backward t-hop
+//    U_t(x-t)^dagger P psi(x-t) of the next slice is computed from the current
+//    slice's own spinor and link and carried in registers, so the t-neighbours
+//    (the re-reads that miss the 4 MB L2 in a site-parallel launch) are never
+//    fetched twice; XCDs are mapped to (t-chunk, z-range) units so one step's
+//    working set stays in the XCD's L2.
+//  * k_site: one work item per site (lattices whose 3-D volume is not a
+//    multiple of 256).
+// SoA fields make every wave-wide load one contiguous segment; periodic /
+// antiperiodic wrap happens in the kernel.  After every half-direction the
+// accumulators are pinned (an empty asm with register operands), which
+// keeps the compiler from sinking the arithmetic below later loads and
+// running out of registers.  The arithmetic follows the reference's half-spin
+// kernels operation by operation:
+//   kernel_WilsonDiracOperator4D! NC=3        src/Operators/WilsonDiracOperator.jl:413-419
+//   kernel_adjoint_WilsonDiracOperator4D!     :514-520
+//   kernel_WilsonDiracOperator4D_Donly!       :624-631, adjoint :736-743
+//   kernel_WilsonDiracCloverOperator4D!       src/Operators/WilsonDiracCloverOperator.jl:537-559
+#pragma once
+
+#include <algorithm>
+
+#include "lmx_internal.h"
+
+namespace lmx {
+
+// ---------------------------------------------------------------------------
+// Clover field storage: 6 anti-Hermitian planes, 9 reals each, as 27 complex
+// slots per site: slots 0..8 carry the 18 imaginary diagonals (plane*3+i) in
+// pairs, slots 9..26 the upper off-diagonals (plane*3 + {01,02,12}).
+// ---------------------------------------------------------------------------
+template <typename R, class Acc>
+__device__ __forceinline__ void clover_add(const Acc& F, int s, int V, const cx<R> psi[12], R coefficient,
+                                           cx<R> res[12]) {
+    R d[18];
+    cx<R> u[18];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        cx<R> v = F.ld((long)q * V + s);
+        d[2 * q] = v.re;
+        d[2 * q + 1] = v.im;
+    }
+#pragma unroll
+    for (int q = 0; q < 18; ++q) u[q] = F.ld((long)(9 + q) * V + s);
+    // kernel_add_clover_term!, src/Operators/WilsonDiracCloverOperator.jl:481-525
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        cx<R> v1 = mk<R>(0, 0), v2 = mk<R>(0, 0), v3 = mk<R>(0, 0), v4 = mk<R>(0, 0);
+#pragma unroll
+        for (int ic = 0; ic < 3; ++ic) {
+            cx<R> f[6];
+#pragma unroll
+            for (int p = 0; p < 6; ++p) {
+                if (c == ic) {
+                    f[p] = mk<R>(R(0), d[3 * p + c]);
+                } else if (c < ic) {
+                    const int j = (c == 0) ? (ic == 1 ? 0 : 1) : 2;
+                    f[p] = u[3 * p + j];
+                } else {
+                    const int j = (ic == 0) ? (c == 1 ? 0 : 1) : 2;
+                    f[p] = cneg(cconj(u[3 * p + j]));
+                }
+            }
+            cx<R> f12 = f[0], f13 = f[1], f14 = f[2], f23 = f[3], f24 = f[4], f34 = f[5];
+            cx<R> s1 = cadd(f12, f34), s2 = csub(f12, f34), s3 = cadd(f14, f23), s4 = csub(f23, f14);
+            cx<R> dp = mk<R>(-s1.im, s1.re), dm = mk<R>(-s2.im, s2.re);
+            cx<R> odp = mk<R>(-s3.im, s3.re), odm = mk<R>(-s4.im, s4.re);
+            cx<R> p1 = psi[ic], p2 = psi[3 + ic], p3 = psi[6 + ic], p4 = psi[9 + ic];
+            v1 = cadd(v1, cadd(cmul(dp, p1), cmul(cadd(cadd(cneg(f13), odp), f24), p2)));
+            v2 = cadd(v2, cadd(cmul(cneg(dp), p2), cmul(csub(cadd(f13, odp), f24), p1)));
+            v3 = cadd(v3, cadd(cmul(dm, p3), cmul(csub(cadd(cneg(f13), odm), f24), p4)));
+            v4 = cadd(v4, cadd(cmul(cneg(dm), p4), cmul(cadd(cadd(f13, odm), f24), p3)));
+        }
+        res[c] = cadd(res[c], rmul(coefficient, v1));
+        res[3 + c] = cadd(res[3 + c], rmul(coefficient, v2));
+        res[6 + c] = cadd(res[6 + c], rmul(coefficient, v3));
+        res[9 + c] = cadd(res[9 + c], rmul(coefficient, v4));
+    }
+}
+
+struct F64Clover {
+    const double2* __restrict__ p;
+    __device__ __forceinline__ cx<double> ld(long i) const {
+        double2 v = p[i];
+        return mk<double>(v.x, v.y);
+    }
+};
+struct F32Clover {
+    const float2* __restrict__ p;
+    __device__ __forceinline__ cx<float> ld(long i) const {
+        float2 v = p[i];
+        return mk<float>(v.x, v.y);
+    }
+};
+struct F16Clover {
+    const __half2* __restrict__ p;
+    __device__ __forceinline__ cx<float> ld(long i) const {
+        float2 v = __half22float2(p[i]);
+        return mk<float>(v.x, v.y);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Per-half-direction arithmetic.  PM is the projector sign of this half.
+// ---------------------------------------------------------------------------
+template <int NR, typename R>
+__device__ __forceinline__ void pin_all(cx<R> (&a)[NR][12]) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) pin_acc(a[r]);
+}
+
+template <int MU, int PM, bool FWD, int NR, bool GP, class In, class Lk, typename R>
+__device__ __forceinline__ void hop_half(cx<R> (&acc)[NR][12], const In& in, const Lk& U, const Geom& g, int s,
+                                         const int x[4], const int (&rr)[NR]) {
+    bool w;
+    const int t = FWD ? nbr_fwd(s, x, MU, g, w) : nbr_bwd(s, x, MU, g, w);
+    const bool ng = w && g.phase_neg[MU];
+    cx<R> u[9];
+    U.template ld9<R>(MU, FWD ? s : t, u);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        cx<R> psi[12], h[6], o[6];
+        in.ld12(rr[r], t, psi);
+        if (GP) {
+            if (w && !g.phase_one[MU]) {
+                if (FWD)
+                    phase_spinor(psi, g.ph_re[MU], g.ph_im[MU]);
+                else
+                    phase_spinor(psi, g.iph_re[MU], g.iph_im[MU]);
+            }
+        }
+        half_project<PM, MU>(psi, h);
+        if (!GP) neg_select(h, ng);
+        if (FWD)
+            matvec_fwd(u, h, o);
+        else
+            matvec_bwd(u, h, o);
+        reconstruct_add<PM, MU>(acc[r], o);
+    }
+    pin_all(acc);
+}
+
+struct EpiArgs {
+    double* partials;  // block partials of Re <aux, out>, row r at partials + r * pstride
+    long pstride;
+    CGState* st;       // early exit when st->done
+    const int* gate;   // early exit while *gate == 0
+    const int* rdone;  // per-RHS retirement flags (stride rdone_stride ints), or null
+    int rdone_stride;
+};
+
+enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
+enum { EPI_NONE = 0, EPI_PAP = 1 };
+
+// Finish one site from its own spinor psi: out = psi - kappa*hop (+ clover),
+// or 1/2 hop; optional Re <aux, out> for the fused CG reduction.
+template <int MODE, int EPI, class Out, class Cl, class Aux, typename R>
+__device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)[12], const Cl& F, const Aux& aux,
+                                              int r, int s, const Geom& g, R kappa, R ccoef, const cx<R> (&acc)[12],
+                                              bool store) {
+    cx<R> res[12];
+    if (MODE == MODE_DONLY) {
+        const R half = R(1) / R(2);
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) res[sp * 3 + c] = cadd(mk<R>(R(0), R(0)), rmul(half, acc[c * 4 + sp]));
+    } else {
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) res[sp * 3 + c] = cadd(psi[sp * 3 + c], rmul(-kappa, acc[c * 4 + sp]));
+        if constexpr (MODE == MODE_CLOVER) clover_add<R>(F, s, g.V, psi, ccoef, res);
+    }
+    if (store) out.st12(r, s, res);
+    double red = 0.0;
+    if constexpr (EPI == EPI_PAP) {
+        // Re sum conj(p) * Ap over the site, in kernel_dot_D order (spin, colour)
+        cx<R> pv[12];
+        aux.ld12(r, s, pv);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            cx<double> t = cmul(cconj(mk<double>(double(pv[k].re), double(pv[k].im))),
+                                mk<double>(double(res[k].re), double(res[k].im)));
+            red = red + t.re;
+        }
+    }
+    return red;
+}
+
+template <int NR>
+__device__ __forceinline__ void rhs_rows(int rg, int nrhs, int (&rr)[NR], int& nr) {
+    const int r0 = rg * NR;
+    nr = min(NR, nrhs - r0);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) rr[r] = r0 + (r < nr ? r : 0);  // ragged group: recompute, do not store
+}
+
+// ---------------------------------------------------------------------------
+// Per-site kernel
+// ---------------------------------------------------------------------------
+template <class In, class Lk, class Out, class Cl, class Aux, int FPM, int MODE, int NR, int EPI, bool GP>
+__global__ void __launch_bounds__(256) k_site(Out out, In in, Lk U, Cl F, Aux aux, Geom g, double kappa_d,
+                                              double ccoef_d, int ngroups, int nrhs, EpiArgs epi) {
+    using R = typename In::R;
+    if (epi.st != nullptr && epi.st->done) return;
+    if (epi.gate != nullptr && *epi.gate == 0) return;
+    __shared__ double lds4[4];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int sb = lb / ngroups, rg = lb - sb * ngroups;
+    const int s = sb * 256 + threadIdx.x;
+    int rr[NR], nr;
+    rhs_rows<NR>(rg, nrhs, rr, nr);
+    double red[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) red[r] = 0.0;
+    if (s < g.V) {
+        int x[4];
+        site_coords(s, g, x);
+        cx<R> acc[NR][12];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int k = 0; k < 12; ++k) acc[r][k] = mk<R>(R(0), R(0));
+        hop_half<0, FPM, true, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<0, -FPM, false, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<1, FPM, true, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<1, -FPM, false, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<2, FPM, true, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<2, -FPM, false, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<3, FPM, true, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<3, -FPM, false, NR, GP>(acc, in, U, g, s, x, rr);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            cx<R> psi[12];
+            if (MODE != MODE_DONLY) in.ld12(rr[r], s, psi);
+            red[r] = finish_site<MODE, EPI>(out, psi, F, aux, rr[r], s, g, R(kappa_d), R(ccoef_d), acc[r], r < nr);
+        }
+    }
+    if constexpr (EPI == EPI_PAP) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double tot = block_tree(red[r], lds4);
+            if (r < nr && threadIdx.x == 0) epi.partials[rr[r] * epi.pstride + sb] = tot;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// t-streaming kernel (3-D volume a multiple of 256)
+// ---------------------------------------------------------------------------
+template <class In, class Lk, class Out, class Cl, class Aux, int FPM, int MODE, int NR, int EPI, bool GP>
+__global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux aux, Geom g, double kappa_d,
+                                                double ccoef_d, int ngroups, int nrhs, int TL, int ZS, EpiArgs epi) {
+    using R = typename In::R;
+    if (epi.st != nullptr && epi.st->done) return;
+    if (epi.gate != nullptr && *epi.gate == 0) return;
+    __shared__ double lds4[4];
+    const int V3 = g.stride[3];
+    const int nblk3 = V3 >> 8;
+    const int ntc = g.L[3] / TL;
+    const int per = nblk3 / ZS;
+    int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int rg = lb % ngroups;
+    lb /= ngroups;
+    const int b3lo = lb % per;
+    lb /= per;
+    const int tc = lb % ntc;
+    const int b3 = (lb / ntc) * per + b3lo;
+    const int s3 = (b3 << 8) + threadIdx.x;
+    int rr[NR], nr;
+    rhs_rows<NR>(rg, nrhs, rr, nr);
+    int x[4];
+    x[0] = s3 % g.L[0];
+    x[1] = (s3 / g.L[0]) % g.L[1];
+    x[2] = s3 / g.stride[2];
+    const int t0 = tc * TL;
+    const R kappa = R(kappa_d), ccoef = R(ccoef_d);
+    // carried backward t-hop of the current slice: B = U_t(x-t)^dag P psi(x-t)
+    cx<R> B[NR][6];
+    {
+        x[3] = t0;
+        const int s = s3 + V3 * t0;
+        bool w;
+        const int tm = nbr_bwd(s, x, 3, g, w);
+        const bool ng = w && g.phase_neg[3];
+        cx<R> u[9];
+        U.template ld9<R>(3, tm, u);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            cx<R> psi[12], h[6];
+            in.ld12(rr[r], tm, psi);
+            if (GP) {
+                if (w && !g.phase_one[3]) phase_spinor(psi, g.iph_re[3], g.iph_im[3]);
+            }
+            half_project<-FPM, 3>(psi, h);
+            if (!GP) neg_select(h, ng);
+            matvec_bwd(u, h, B[r]);
+        }
+    }
+    for (int stp = 0; stp < TL; ++stp) {
+        const int t = t0 + stp;
+        x[3] = t;
+        const int s = s3 + V3 * t;
+        cx<R> acc[NR][12];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int k = 0; k < 12; ++k) acc[r][k] = mk<R>(R(0), R(0));
+        hop_half<0, FPM, true, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<0, -FPM, false, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<1, FPM, true, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<1, -FPM, false, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<2, FPM, true, NR, GP>(acc, in, U, g, s, x, rr);
+        hop_half<2, -FPM, false, NR, GP>(acc, in, U, g, s, x, rr);
+        // t forward from memory, t backward from the carried B (reference order: forward, then backward)
+        cx<R> ut[9];
+        {
+            bool w;
+            const int tp = nbr_fwd(s, x, 3, g, w);
+            const bool ng = w && g.phase_neg[3];
+            U.template ld9<R>(3, s, ut);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                cx<R> psi[12], h[6], o[6];
+                in.ld12(rr[r], tp, psi);
+                if (GP) {
+                    if (w && !g.phase_one[3]) phase_spinor(psi, g.ph_re[3], g.ph_im[3]);
+                }
+                half_project<FPM, 3>(psi, h);
+                if (!GP) neg_select(h, ng);
+                matvec_fwd(ut, h, o);
+                reconstruct_add<FPM, 3>(acc[r], o);
+                reconstruct_add<-FPM, 3>(acc[r], B[r]);
+            }
+        }
+        double red[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            // this slice's own spinor: the output's diagonal term and the next
+            // slice's backward t-hop
+            cx<R> p[12], h[6];
+            in.ld12(rr[r], s, p);
+            red[r] = finish_site<MODE, EPI>(out, p, F, aux, rr[r], s, g, kappa, ccoef, acc[r], r < nr);
+            half_project<-FPM, 3>(p, h);
+            matvec_bwd(ut, h, B[r]);
+        }
+        if constexpr (EPI == EPI_PAP) {
+            const int blk = b3 + nblk3 * t;  // the 256 lexicographic sites this step covered
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const double tot = block_tree(red[r], lds4);
+                if (r < nr && threadIdx.x == 0) epi.partials[rr[r] * epi.pstride + blk] = tot;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch shape and dispatch
+// ---------------------------------------------------------------------------
+struct Shape {
+    bool stream;
+    int TL, ZS;
+};
+
+// t-streaming when the 3-D volume splits into 256-site blocks; TL and ZS keep
+// enough workgroups for the chip and keep one step's working set per XCD
+// small.  A context override (tune_tl / tune_zs) wins when it divides.
+inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks) {
+    Shape sh{false, 1, 1};
+    const Geom& g = ctx->g;
+    const long V3 = (long)g.L[0] * g.L[1] * g.L[2];
+    if (V3 % 256 != 0) return sh;
+    const int nblk3 = (int)(V3 / 256);
+    sh.stream = true;
+    if (ctx->tune_tl > 0 && g.L[3] % ctx->tune_tl == 0) {
+        sh.TL = ctx->tune_tl;
+    } else {
+        const int cand[5] = {16, 8, 4, 2, 1};
+        for (int c : cand) {
+            if (g.L[3] % c) continue;
+            if ((long)nblk3 * (g.L[3] / c) * ngroups >= min_blocks || c == 1) {
+                sh.TL = c;
+                break;
+            }
+        }
+    }
+    if (ctx->tune_zs > 0 && nblk3 % ctx->tune_zs == 0) {
+        sh.ZS = ctx->tune_zs;
+    } else {
+        const int zs_cand[3] = {4, 2, 1};
+        for (int z : zs_cand)
+            if (nblk3 % z == 0) {
+                sh.ZS = z;
+                break;
+            }
+    }
+    return sh;
+}
+
+// GP (complex boundary phases) runs in the per-site kernel only; SITE_NG
+// instantiates the per-site kernel for +-1 phases too (else the GP kernel
+// covers lattices whose 3-D volume is not a multiple of 256).  EPI_PAP
+// (fused Re <p, Ap>) exists for D^dagger, the second half of D^dagger D.
+template <class In, class Lk, class Out, class Cl, class Aux, int NR, int EPI, bool SITE_NG, bool DONLY_OK>
+int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, double kappa, double ccoef, int ngroups, int nrhs,
+             int flags, bool clover, const EpiArgs& epi, int min_blocks) {
+    const bool dag = (flags & LMX_DAGGER) != 0;
+    const bool donly = (flags & LMX_DONLY) != 0;
+    const Shape sh = pick_shape(ctx, ngroups, min_blocks);
+    const int nb_site = nblocks(ctx->V) * ngroups;
+    const int nb_stream = (int)(ctx->V / 256 / sh.TL) * ngroups;
+    const bool general = ctx->g.general != 0;
+    hipStream_t st = ctx->stream;
+    if (EPI == EPI_PAP && (!dag || donly)) {
+        set_error("the fused p.Ap reduction exists for D^dagger only");
+        return LMX_ERR_UNSUPPORTED;
+    }
+    if (donly && !DONLY_OK) {
+        set_error("hopping-only operator is built in FP64 only");
+        return LMX_ERR_UNSUPPORTED;
+    }
+#define LMX_GO(FPM, MODE)                                                                                             \
+    do {                                                                                                              \
+        if (general || (!sh.stream && !SITE_NG))                                                                      \
+            hipLaunchKernelGGL((k_site<In, Lk, Out, Cl, Aux, FPM, MODE, NR, EPI, true>), dim3(nb_site), dim3(256), 0, \
+                               st, out, in, U, F, aux, ctx->g, kappa, ccoef, ngroups, nrhs, epi);                     \
+        else if (sh.stream)                                                                                           \
+            hipLaunchKernelGGL((k_stream<In, Lk, Out, Cl, Aux, FPM, MODE, NR, EPI, false>), dim3(nb_stream),         \
+                               dim3(256), 0, st, out, in, U, F, aux, ctx->g, kappa, ccoef, ngroups, nrhs, sh.TL,     \
+                               sh.ZS, epi);                                                                           \
+        else if constexpr (SITE_NG)                                                                                   \
+            hipLaunchKernelGGL((k_site<In, Lk, Out, Cl, Aux, FPM, MODE, NR, EPI, false>), dim3(nb_site), dim3(256),  \
+                               0, st, out, in, U, F, aux, ctx->g, kappa, ccoef, ngroups, nrhs, epi);                  \
+    } while (0)
+    if constexpr (EPI == EPI_PAP) {
+        if (clover)
+            LMX_GO(1, MODE_CLOVER);
+        else
+            LMX_GO(1, MODE_WILSON);
+    } else {
+        if (donly) {
+            if constexpr (DONLY_OK) {
+                if (dag)
+                    LMX_GO(1, MODE_DONLY);
+                else
+                    LMX_GO(-1, MODE_DONLY);
+            }
+        } else if (clover) {
+            if (dag)
+                LMX_GO(1, MODE_CLOVER);
+            else
+                LMX_GO(-1, MODE_CLOVER);
+        } else {
+            if (dag)
+                LMX_GO(1, MODE_WILSON);
+            else
+                LMX_GO(-1, MODE_WILSON);
+        }
+    }
+#undef LMX_GO
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+// One low-precision (In, Out, Lk, Cl, NR) combination: rhs_block >> 4
+// right-hand sides per launch, NR per work item; fused Re <aux, out> when
+// a.partials is set.  Instantiated per combination in wilson_lo.hip.
+template <class In, class Out, class Lk, class Cl, int NR>
+int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
+    const double ccoef = (-a.kappa) * a.csw;
+    int per_launch = a.rhs_block >> 4;
+    if (per_launch <= 0) per_launch = 1;
+    const int V = (int)ctx->V;
+    for (int r0 = 0; r0 < a.nrhs; r0 += per_launch) {
+        const int n = std::min(per_launch, a.nrhs - r0);
+        const int ngroups = (n + NR - 1) / NR;
+        Out o = Out::at(a.out, a.out_norm, V, r0);
+        In i = In::at(a.in, a.in_norm, V, r0);
+        EpiArgs epi{a.partials ? a.partials + (long)r0 * a.pstride : nullptr, a.pstride, nullptr, a.gate, nullptr, 0};
+        if (a.partials) {
+            In x = In::at(a.aux, a.aux_norm, V, r0);
+            LMX_TRY((dispatch<In, Lk, Out, Cl, In, NR, EPI_PAP, false, false>(ctx, o, i, U, F, x, a.kappa, ccoef,
+                                                                               ngroups, n, a.flags, clover, epi, 512)));
+        } else {
+            LMX_TRY((dispatch<In, Lk, Out, Cl, In, NR, EPI_NONE, false, false>(ctx, o, i, U, F, i, a.kappa, ccoef,
+                                                                                ngroups, n, a.flags, clover, epi, 512)));
+        }
+    }
+    return LMX_OK;
+}
+
+// Low-precision combinations built: (spinor, links) = (F32, F32), (F32, F16), (H16, F16).
+#define LMX_LO_COMBO_0(X, NR) X(SpF32In, SpF32Out, LinkF32, F32Clover, NR)
+#define LMX_LO_COMBO_1(X, NR) X(SpF32In, SpF32Out, LinkF16, F16Clover, NR)
+#define LMX_LO_COMBO_2(X, NR) X(SpH16In, SpH16Out, LinkF16, F16Clover, NR)
+#define LMX_LO_COMBOS(X, NR) LMX_LO_COMBO_0(X, NR) LMX_LO_COMBO_1(X, NR) LMX_LO_COMBO_2(X, NR)
+#define LMX_LO_ALL(X) LMX_LO_COMBOS(X, 1) LMX_LO_COMBOS(X, 2) LMX_LO_COMBOS(X, 4)
+
+}  // namespace lmx

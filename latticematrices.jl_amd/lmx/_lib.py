@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblmx.so")
 
 LMX_OK, LMX_ERR_ARG, LMX_ERR_ALIAS, LMX_ERR_HIP, LMX_ERR_NOMEM, LMX_ERR_UNSUPPORTED, LMX_ERR_WORKSPACE = range(7)
-LMX_F64, LMX_F32, LMX_F16 = 0, 1, 2
+LMX_F64, LMX_F32, LMX_F16, LMX_H16 = 0, 1, 2, 3
 LMX_DAGGER, LMX_DONLY = 1, 2
 LMX_CG_CONVERGED, LMX_CG_MAXITER, LMX_CG_NONPOSITIVE_CURVATURE, LMX_CG_BREAKDOWN = 0, 1, 2, 3
 REASONS = {0: "converged", 1: "maximum_iterations", 2: "nonpositive_curvature", 3: "numerical_breakdown"}

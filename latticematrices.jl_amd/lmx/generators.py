@@ -57,6 +57,45 @@ def su3_exp_i_hermitian(H: np.ndarray) -> np.ndarray:
     return (v * np.exp(1j * w)[..., None, :]) @ np.conj(np.swapaxes(v, -1, -2))
 
 
+def su3_exp_i_traceless(Q: np.ndarray) -> np.ndarray:
+    """exp(i Q) for traceless Hermitian Q[..., 3, 3] in closed form:
+    exp(iQ) = f0 + f1 Q + f2 Q^2 with the Cayley-Hamilton coefficients of
+    Morningstar & Peardon, Phys. Rev. D 69, 054501 (2004), eqs. (23)-(34)."""
+    Q2 = Q @ Q
+    c0 = (np.einsum("...ij,...ji->...", Q2, Q).real) / 3.0          # det Q
+    c1 = 0.5 * np.einsum("...ii->...", Q2).real                       # tr Q^2 / 2
+    neg = c0 < 0
+    c0a = np.abs(c0)
+    c0max = 2.0 * (c1 / 3.0) ** 1.5
+    ratio = np.where(c0max > 0, c0a / np.where(c0max > 0, c0max, 1.0), 0.0)
+    theta = np.arccos(np.clip(ratio, -1.0, 1.0))
+    u = np.sqrt(c1 / 3.0) * np.cos(theta / 3.0)
+    w = np.sqrt(c1) * np.sin(theta / 3.0)
+    w2 = w * w
+    small = np.abs(w) < 0.05
+    xi0 = np.where(small, 1 - w2 / 6 * (1 - w2 / 20 * (1 - w2 / 42)), np.sin(w) / np.where(small, 1.0, w))
+    e2 = np.exp(2j * u)
+    em = np.exp(-1j * u)
+    cw = np.cos(w)
+    h0 = (u * u - w2) * e2 + em * (8 * u * u * cw + 2j * u * (3 * u * u + w2) * xi0)
+    h1 = 2 * u * e2 - em * (2 * u * cw - 1j * (3 * u * u - w2) * xi0)
+    h2 = e2 - em * (cw + 3j * u * xi0)
+    den = 9 * u * u - w2
+    tiny = den < 1e-12
+    den = np.where(tiny, 1.0, den)
+    f0, f1, f2 = h0 / den, h1 / den, h2 / den
+    # c0 < 0: f_j(-c0) = (-1)^j conj(f_j(c0))
+    f0 = np.where(neg, np.conj(f0), f0)
+    f1 = np.where(neg, -np.conj(f1), f1)
+    f2 = np.where(neg, np.conj(f2), f2)
+    # Q -> 0: second-order Taylor (|Q| below 1e-6 only)
+    f0 = np.where(tiny, 1.0 - c1, f0)
+    f1 = np.where(tiny, 1.0 + 0j, f1)
+    f2 = np.where(tiny, -0.5 + 0j, f2)
+    I = np.eye(3, dtype=np.complex128)
+    return f0[..., None, None] * I + f1[..., None, None] * Q + f2[..., None, None] * Q2
+
+
 def weak_field_su3(n: int, eps: float, seed: int, chunk: int = 1 << 20) -> np.ndarray:
     """n weak-field SU(3) links U = exp(i eps H), H = sum_a h_a lambda_a / 2
     with h_a ~ N(0,1) (traceless Hermitian Gaussian).  complex128 [n, 3, 3]."""
@@ -66,7 +105,7 @@ def weak_field_su3(n: int, eps: float, seed: int, chunk: int = 1 << 20) -> np.nd
         m = min(chunk, n - s)
         h = rng.standard_normal((m, 8))
         H = np.einsum("na,aij->nij", h, _GM) * 0.5
-        out[s:s + m] = su3_exp_i_hermitian(eps * H)
+        out[s:s + m] = su3_exp_i_traceless(eps * H)
     return out
 
 

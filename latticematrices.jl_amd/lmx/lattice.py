@@ -453,7 +453,7 @@ def ta_project_(F: GaugeField, U: GaugeField, G: GaugeField) -> GaugeField:
     return F
 
 
-_PREC = {"f64": _lib.LMX_F64, "f32": _lib.LMX_F32, "f16": _lib.LMX_F16}
+_PREC = {"f64": _lib.LMX_F64, "f32": _lib.LMX_F32, "f16": _lib.LMX_F16, "h16": _lib.LMX_H16}
 
 
 def cg_batched_(x: SpinorField, D: WilsonDiracOperator4D, b: SpinorField, precision: str = "f32", rtol=1e-10,
@@ -461,12 +461,14 @@ def cg_batched_(x: SpinorField, D: WilsonDiracOperator4D, b: SpinorField, precis
                 use_graph=False, rhs_block=0, workspace=None) -> list:
     """Solve D^dagger D x_i = b_i for the x.nrhs right-hand sides at once.
 
-    precision: "f64" (plain cg! per RHS), "f32" (FP32 inner CG) or "f16"
-    (FP16-stored links, FP32 spinors and math); the mixed modes use FP64
+    precision: "f64" (plain cg! per RHS), "f32" (FP32 inner CG), "f16"
+    (FP16-stored links, FP32 spinors and math) or "h16" (FP16 links, inner
+    vectors as 16-bit fixed point with a per-site scale, FP32 math; x_inner
+    stays FP32); the mixed modes use FP64
     reliable updates, FP64 reductions and FP64 solution accumulation, and
     stop on the FP64 true residual.  x holds the initial guess.  rhs_block
-    encodes (RHS per launch) << 4 | (RHS per work item); 0 uses the
-    autotuned or default shape.  Returns one CGResult per RHS."""
+    encodes (RHS per launch) << 4 | (RHS per work item, 1, 2 or 4); 0 uses
+    the autotuned or default shape.  Returns one CGResult per RHS."""
     if x.nrhs != b.nrhs:
         raise ArgumentError(_lib.LMX_ERR_ARG, "x and b must hold the same number of right-hand sides")
     if x.data.data_ptr() == b.data.data_ptr():

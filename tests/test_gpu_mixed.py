@@ -34,7 +34,7 @@ def oracle_check(U, L, kappa, b, x, rtol, csw=0.0):
     return out
 
 
-@pytest.mark.parametrize("precision", ["f32", "f16"])
+@pytest.mark.parametrize("precision", ["f32", "f16", "h16"])
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_mixed_batched_converges_to_fp64_true_residual(gpu, precision, use_graph):
     L, kappa, rtol = (8, 8, 8, 8), 0.124, 1e-10
@@ -65,7 +65,7 @@ def test_graph_and_eager_agree(gpu):
     assert np.array_equal(x1.to_numpy(), x2.to_numpy())
 
 
-@pytest.mark.parametrize("rhs_block", [(1 << 4) | 1, (4 << 4) | 2, (5 << 4) | 1, (3 << 4) | 3, (4 << 4) | 4])
+@pytest.mark.parametrize("rhs_block", [(1 << 4) | 1, (4 << 4) | 2, (5 << 4) | 1, (3 << 4) | 2, (4 << 4) | 4])
 def test_rhs_blocking_uneven_batch(gpu, rhs_block):
     # 5 RHS split unevenly by the launch shape; every shape gives the same iterates
     L, kappa = (4, 4, 4, 8), 0.12
@@ -123,4 +123,22 @@ def test_mixed_clover(gpu):
     checks = oracle_check(U, L, kappa, b, x.to_numpy(), rtol, csw=csw)
     for r, (tr, err) in zip(res, checks):
         print(f"  clover iters={r.iterations} true_rel={tr:.3e} err={err:.3e}", flush=True)
+        assert r.converged and tr <= rtol * 1.05 and err <= 1e-7
+
+
+@pytest.mark.parametrize("precision", ["f16", "h16"])
+def test_mixed_stream_shape(gpu, monkeypatch, precision):
+    # forced multi-slice streaming shape with the fused p.Ap on D^dagger,
+    # 4 RHS at 2 per work item, graph replay
+    monkeypatch.setenv("LMX_STREAM_TL", "4")
+    monkeypatch.setenv("LMX_STREAM_ZS", "2")
+    L, kappa, rtol = (8, 8, 8, 8), 0.124, 1e-10
+    U, b, lat, Ug, D = setup(L, "weak", 360, kappa, nrhs=4)
+    x = lat.spinor(4)
+    res = lmx.cg_batched_(x, D, lat.spinor_from_numpy(b), precision=precision, rtol=rtol, use_graph=True,
+                          rhs_block=(4 << 4) | 2)
+    checks = oracle_check(U, L, kappa, b, x.to_numpy(), rtol)
+    for r, (tr, err) in zip(res, checks):
+        print(f"  {precision} iters={r.iterations} updates={r.reliable_updates} true_rel={tr:.3e} err={err:.3e}",
+              flush=True)
         assert r.converged and tr <= rtol * 1.05 and err <= 1e-7

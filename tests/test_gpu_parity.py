@@ -30,6 +30,10 @@ CASES = [
     ((4, 4, 4, 4), "haar", (1, 1, 1, -1)),
     ((3, 2, 2, 4), "rand", (np.exp(0.3j), 1, -1j, -1)),
     ((8, 4, 6, 2), "haar", (-1, 1, 1, -1)),
+    # 3-D volume a multiple of 256: the t-streaming kernel (auto shape
+    # TL = 2 t-slices per workgroup, 4 z-ranges at 16^3 x 32)
+    ((8, 8, 4, 8), "rand", (-1, 1, 1, -1)),
+    ((16, 16, 16, 32), "haar", (1, 1, 1, -1)),
 ]
 
 
@@ -216,3 +220,77 @@ def test_pullback_matches_oracle(gpu):
     lmx.wilson_pullback_(dU, D, lat.spinor_from_numpy(w), lat.spinor_from_numpy(psi), -0.125)
     ref = orc.wilson_pullback(w, psi, L, -0.125)
     assert np.array_equal(dU.to_numpy(), ref)
+
+
+# Forced streaming shapes (t-slices per workgroup TL, 3-D sub-ranges ZS) on
+# small lattices: every shape must give the same bits as the oracle.
+STREAM_SHAPES = [((8, 8, 4, 16), 4, 1), ((16, 16, 8, 8), 8, 2), ((16, 8, 8, 12), 3, 4), ((8, 8, 8, 4), 4, 2)]
+
+
+@pytest.mark.parametrize("L,TL,ZS", STREAM_SHAPES)
+def test_stream_shapes_bit_exact(gpu, monkeypatch, L, TL, ZS):
+    print(f"stream shape L={L} TL={TL} ZS={ZS}", flush=True)
+    monkeypatch.setenv("LMX_STREAM_TL", str(TL))
+    monkeypatch.setenv("LMX_STREAM_ZS", str(ZS))
+    phases = (1, -1, 1, -1)
+    U = gen.links(L, "haar", 21)
+    psi = gen.gaussian_spinor(L, 22)
+    lat = lmx.Lattice(L, phases)
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracOperator4D(Ug, 0.121)
+    x = lat.spinor_from_numpy(psi)
+    y = lat.spinor()
+    for flags in (0, 1, 2, 3):
+        A = lmx.WilsonDiracOperator4D_Donly(Ug) if flags & 2 else D
+        if flags & 1:
+            A = lmx.adjoint(A)
+        lmx.mul_(y, A, x)
+        ref = orc.wilson(U, psi, L, 0.121, phases, dagger=bool(flags & 1), donly=bool(flags & 2))
+        assert np.array_equal(y.to_numpy(), ref), f"flags={flags} max rel {rel(y.to_numpy(), ref):.3e}"
+    # clover on the same shape
+    Dc = lmx.WilsonDiracCloverOperator4D(Ug, 0.121, 1.3)
+    F = orc.clover_field_strength(U, L)
+    for dag in (False, True):
+        lmx.mul_(y, lmx.adjoint(Dc) if dag else Dc, x)
+        ref = orc.wilson_clover(U, F, psi, L, 0.121, 1.3, dagger=dag, phases=phases)
+        assert np.array_equal(y.to_numpy(), ref), f"clover dag={dag}"
+
+
+def test_cg_stream_path_bit_exact(gpu, monkeypatch):
+    # CG with the fused p.Ap epilogue in the streaming kernel (block partials
+    # indexed by the 256 lexicographic sites each step covers)
+    monkeypatch.setenv("LMX_STREAM_TL", "4")
+    monkeypatch.setenv("LMX_STREAM_ZS", "2")
+    L = (16, 8, 4, 8)
+    U = gen.links(L, "haar", 61)
+    b = gen.gaussian_spinor(L, 62)
+    for csw in (0.0, 1.0):
+        F = orc.clover_field_strength(U, L) if csw else None
+        op = orc.NormalOp(orc.OP_DDAGD_CLOVER if csw else orc.OP_DDAGD_WILSON, L, U=U, F=F, kappa=0.12, csw=csw)
+        xo, ro = orc.cg(op, b, rtol=1e-10, order=1)
+        lat = lmx.Lattice(L, (1, 1, 1, -1))
+        D = lmx.WilsonDiracOperator4D(lat.gauge_from_numpy(U), 0.12, csw)
+        bg = lat.spinor_from_numpy(b)
+        x, r, p, Ap, tmp = (lat.spinor() for _ in range(5))
+        res = lmx.cg_(x, lmx.DdagDOp(D, tmp), bg, r, p, Ap, rtol=1e-10)
+        print(f"  csw={csw} iterations gpu={res.iterations} oracle={ro.iterations}", flush=True)
+        assert res.converged and res.iterations == ro.iterations
+        assert np.array_equal(x.to_numpy(), xo)
+        assert res.residual_norm == ro.residual_norm
+
+
+def test_full_volume_bit_exact(gpu):
+    # the benchmark volume 32^4 (automatic shape: 16 t-slices per workgroup,
+    # 4 z-ranges), weak-field links eps = 0.2, kappa = 0.124
+    L = (32, 32, 32, 32)
+    print("full volume 32^4 D and D^dagger", flush=True)
+    U = gen.links(L, "weak", 11, eps=0.2)
+    psi = gen.gaussian_spinor(L, 12)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    D = lmx.WilsonDiracOperator4D(lat.gauge_from_numpy(U), 0.124)
+    x = lat.spinor_from_numpy(psi)
+    y = lat.spinor()
+    for dag in (False, True):
+        lmx.mul_(y, lmx.adjoint(D) if dag else D, x)
+        ref = orc.wilson(U, psi, L, 0.124, dagger=dag)
+        assert np.array_equal(y.to_numpy(), ref), f"dagger={dag} max rel {rel(y.to_numpy(), ref):.3e}"
