@@ -83,7 +83,8 @@ typedef struct {
     int32_t reliable_maxiter; /* ... or after this many inner iterations */
     int32_t check_every;      /* iterations between host convergence polls */
     int32_t use_graph;        /* capture check_every iterations in a hipGraph */
-    int32_t rhs_block;        /* right-hand sides per dslash launch (0: autotuned) */
+    int32_t rhs_block;        /* inner dslash shape (0: autotuned): (TL << 12) | (RHS per launch << 4) | RHS per
+                                 work item (1, 2, 4); TL = t-slices per streaming workgroup, 0 automatic */
     int32_t reserved[4];
 } lmx_cg_params;
 
@@ -244,7 +245,7 @@ int lmx_wilson_force(lmx_wilson* op, void* G, void* F, const void* phi, void* et
 /* ======================================================================= */
 
 typedef struct {
-    int32_t rhs_block;   /* chosen RHS block */
+    int32_t rhs_block;   /* chosen shape, encoded as lmx_cg_params.rhs_block */
     int32_t block_size;  /* threads per workgroup */
     double ms_per_apply; /* measured D apply time of the chosen shape, per RHS */
 } lmx_tune_result;

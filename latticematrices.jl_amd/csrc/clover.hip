@@ -162,8 +162,32 @@ static int convert(lmx_context* ctx, const double2* in, float2* o32, __half2* o1
     return LMX_OK;
 }
 
+// FP16 links packed per direction: entries 0-3 and 4-7 as two 16-byte words
+// (int4 [4][2][V]), entry 8 as one 4-byte word (half2 [4][V]) after them.
+__global__ void __launch_bounds__(256) k_pack_links16(const double2* __restrict__ U, int4* __restrict__ P, int V) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int mu = blockIdx.y;
+    if (s >= V) return;
+    int w[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        double2 v = U[((size_t)mu * 9 + k) * V + s];
+        __half2 h = __float22half2_rn(make_float2((float)v.x, (float)v.y));
+        w[k] = *reinterpret_cast<int*>(&h);
+    }
+    P[((size_t)2 * mu) * V + s] = make_int4(w[0], w[1], w[2], w[3]);
+    P[((size_t)2 * mu + 1) * V + s] = make_int4(w[4], w[5], w[6], w[7]);
+    reinterpret_cast<int*>(P + (size_t)8 * V)[(size_t)mu * V + s] = w[8];
+}
+
 int launch_convert_links(lmx_context* ctx, const double2* U, float2* U32, __half2* U16) {
-    return convert(ctx, U, U32, U16, 36 * ctx->V);
+    LMX_TRY(convert(ctx, U, U32, nullptr, 36 * ctx->V));
+    if (U16 && ctx->V > 0) {
+        hipLaunchKernelGGL(k_pack_links16, dim3(nblocks(ctx->V), 4), dim3(256), 0, ctx->stream, U, (int4*)U16,
+                           (int)ctx->V);
+        LMX_HIP(hipGetLastError());
+    }
+    return LMX_OK;
 }
 
 int launch_convert_clover(lmx_context* ctx, const double2* F, float2* F32, __half2* F16) {

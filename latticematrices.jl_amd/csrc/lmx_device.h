@@ -162,6 +162,15 @@ __device__ __forceinline__ void st_out(float* p, float v) {
 #endif
 }
 
+// Loads and stores address a field as (uniform base, 32-bit site index):
+// the base (field + component offset) is the same for the whole wave, so
+// the compiler keeps it in scalar registers and each access costs one
+// 32-bit offset computation instead of a 64-bit address.
+template <typename T>
+__device__ __forceinline__ T ldv(const T* ubase, int s) {
+    return *(const T*)((const char*)ubase + (unsigned)s * (unsigned)sizeof(T));
+}
+
 struct SpF64In {
     using R = double;
     const double2* __restrict__ p;
@@ -169,13 +178,19 @@ struct SpF64In {
     static SpF64In at(const void* q, const float*, int V, int r0) {
         return SpF64In{(const double2*)q + (size_t)r0 * 12 * V, V};
     }
-    __device__ __forceinline__ void ld12(int r, int s, cx<double>* z) const {
+    // unscaled components and the site scale (1 here)
+    __device__ __forceinline__ void ld12s(int r, int s, cx<double>* z, double& c) const {
         const double2* q = p + (size_t)r * 12 * V;
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            double2 v = q[k * V + s];
+            double2 v = ldv(q + (size_t)k * V, s);
             z[k] = mk<double>(v.x, v.y);
         }
+        c = 1.0;
+    }
+    __device__ __forceinline__ void ld12(int r, int s, cx<double>* z) const {
+        double c;
+        ld12s(r, s, z, c);
     }
 };
 struct SpF64Out {
@@ -185,7 +200,7 @@ struct SpF64Out {
     __device__ __forceinline__ void st12(int r, int s, const cx<double>* z) const {
         double2* q = p + (size_t)r * 12 * V;
 #pragma unroll
-        for (int k = 0; k < 12; ++k) st_out(q + k * V + s, z[k].re, z[k].im);
+        for (int k = 0; k < 12; ++k) st_out((double2*)((char*)(q + (size_t)k * V) + (unsigned)s * 16u), z[k].re, z[k].im);
     }
 };
 struct SpF32In {
@@ -195,13 +210,18 @@ struct SpF32In {
     static SpF32In at(const void* q, const float*, int V, int r0) {
         return SpF32In{(const float2*)q + (size_t)r0 * 12 * V, V};
     }
-    __device__ __forceinline__ void ld12(int r, int s, cx<float>* z) const {
+    __device__ __forceinline__ void ld12s(int r, int s, cx<float>* z, float& c) const {
         const float2* q = p + (size_t)r * 12 * V;
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            float2 v = q[k * V + s];
+            float2 v = ldv(q + (size_t)k * V, s);
             z[k] = mk<float>(v.x, v.y);
         }
+        c = 1.0f;
+    }
+    __device__ __forceinline__ void ld12(int r, int s, cx<float>* z) const {
+        float c;
+        ld12s(r, s, z, c);
     }
 };
 struct SpF32Out {
@@ -212,7 +232,7 @@ struct SpF32Out {
     __device__ __forceinline__ void st12(int r, int s, const cx<float>* z) const {
         float2* q = p + (size_t)r * 12 * V;
 #pragma unroll
-        for (int k = 0; k < 12; ++k) st_out(q + k * V + s, z[k].re, z[k].im);
+        for (int k = 0; k < 12; ++k) st_out((float2*)((char*)(q + (size_t)k * V) + (unsigned)s * 8u), z[k].re, z[k].im);
     }
 };
 
@@ -233,17 +253,24 @@ struct SpH16In {
     static SpH16In at(const void* q, const float* n, int V, int r0) {
         return SpH16In{(const int4*)q + (size_t)r0 * 3 * V, n + (size_t)r0 * V, V};
     }
-    __device__ __forceinline__ void ld12(int r, int s, cx<float>* z) const {
+    // integer components (as floats) and the site scale: value = z * c
+    __device__ __forceinline__ void ld12s(int r, int s, cx<float>* z, float& c) const {
         const int4* q = p + (size_t)r * 3 * V;
-        const float c = scale[(size_t)r * V + s];
+        c = ldv(scale + (size_t)r * V, s);
 #pragma unroll
         for (int g = 0; g < 3; ++g) {
-            int4 v = q[g * V + s];
-            z[4 * g + 0] = mk<float>(h16_lo(v.x) * c, h16_hi(v.x) * c);
-            z[4 * g + 1] = mk<float>(h16_lo(v.y) * c, h16_hi(v.y) * c);
-            z[4 * g + 2] = mk<float>(h16_lo(v.z) * c, h16_hi(v.z) * c);
-            z[4 * g + 3] = mk<float>(h16_lo(v.w) * c, h16_hi(v.w) * c);
+            int4 v = ldv(q + (size_t)g * V, s);
+            z[4 * g + 0] = mk<float>(h16_lo(v.x), h16_hi(v.x));
+            z[4 * g + 1] = mk<float>(h16_lo(v.y), h16_hi(v.y));
+            z[4 * g + 2] = mk<float>(h16_lo(v.z), h16_hi(v.z));
+            z[4 * g + 3] = mk<float>(h16_lo(v.w), h16_hi(v.w));
         }
+    }
+    __device__ __forceinline__ void ld12(int r, int s, cx<float>* z) const {
+        float c;
+        ld12s(r, s, z, c);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) z[k] = mk<float>(z[k].re * c, z[k].im * c);
     }
 };
 struct SpH16Out {
@@ -267,12 +294,15 @@ struct SpH16Out {
             v.y = h16_pack(z[4 * g + 1].re * inv, z[4 * g + 1].im * inv);
             v.z = h16_pack(z[4 * g + 2].re * inv, z[4 * g + 2].im * inv);
             v.w = h16_pack(z[4 * g + 3].re * inv, z[4 * g + 3].im * inv);
-            st_out(q + g * V + s, v);
+            st_out((int4*)((char*)(q + (size_t)g * V) + (unsigned)s * 16u), v);
         }
-        st_out(scale + (size_t)r * V + s, m / kH16Max);
+        st_out((float*)((char*)(scale + (size_t)r * V) + (unsigned)s * 4u), m / kH16Max);
     }
 };
 
+// Links: FP64 / FP32 as double2 / float2 [4][9][V]; FP16 packed per
+// direction as two 16-byte words (entries 0-3, 4-7) and one 4-byte word
+// (entry 8): int4 [4][2][V] followed by half2 [4][V], three loads a link.
 struct LinkF64 {
     const double2* __restrict__ p;
     int V;
@@ -281,7 +311,7 @@ struct LinkF64 {
         const double2* q = p + (size_t)mu * 9 * V;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            double2 v = q[k * V + s];
+            double2 v = ldv(q + (size_t)k * V, s);
             u[k] = mk<R>(R(v.x), R(v.y));
         }
     }
@@ -294,20 +324,27 @@ struct LinkF32 {
         const float2* q = p + (size_t)mu * 9 * V;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            float2 v = q[k * V + s];
+            float2 v = ldv(q + (size_t)k * V, s);
             u[k] = mk<R>(R(v.x), R(v.y));
         }
     }
 };
+__device__ __forceinline__ float2 h2f(int w) {
+    __half2 h = *reinterpret_cast<__half2*>(&w);
+    return __half22float2(h);
+}
 struct LinkF16 {
-    const __half2* __restrict__ p;
+    const int4* __restrict__ p;  // packed layout above
     int V;
     template <typename R>
     __device__ __forceinline__ void ld9(int mu, int s, cx<R>* u) const {
-        const __half2* q = p + (size_t)mu * 9 * V;
+        const int4 a = ldv(p + (size_t)(2 * mu) * V, s);
+        const int4 b = ldv(p + (size_t)(2 * mu + 1) * V, s);
+        const int c = ldv((const int*)(p + (size_t)8 * V) + (size_t)mu * V, s);
+        const int w[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c};
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            float2 v = __half22float2(q[k * V + s]);
+            float2 v = h2f(w[k]);
             u[k] = mk<R>(R(v.x), R(v.y));
         }
     }
@@ -371,6 +408,37 @@ __device__ __forceinline__ void reconstruct_add(cx<R> acc[12], const cx<R> r[6])
         } else {
             a[2] = csub(a[2], pmr<PM>(h1));
             a[3] = csub(a[3], pmr<PM>(h2));
+        }
+    }
+}
+
+// acc += c * (reconstructed r): the site scale of a 16-bit spinor applied
+// once per half-spinor (c == 1 for FP32/FP64 fields, where fma(1, x, a) is
+// exactly a + x).
+template <typename R>
+__device__ __forceinline__ cx<R> cfma(R c, cx<R> t, cx<R> a) {
+    return mk<R>(fmaR(c, t.re, a.re), fmaR(c, t.im, a.im));
+}
+template <int PM, int MU, typename R>
+__device__ __forceinline__ void reconstruct_add_s(cx<R> acc[12], const cx<R> r[6], R c) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        cx<R> h1 = r[2 * k], h2 = r[2 * k + 1];
+        cx<R>* a = acc + 4 * k;
+        a[0] = cfma(c, h1, a[0]);
+        a[1] = cfma(c, h2, a[1]);
+        if (MU == 0) {
+            a[2] = cfma(c, pmi<PM>(h2), a[2]);
+            a[3] = cfma(c, pmi<PM>(h1), a[3]);
+        } else if (MU == 1) {
+            a[2] = cfma(c, pmr<PM>(h2), a[2]);
+            a[3] = cfma(c, cneg(pmr<PM>(h1)), a[3]);
+        } else if (MU == 2) {
+            a[2] = cfma(c, pmi<PM>(h1), a[2]);
+            a[3] = cfma(c, cneg(pmi<PM>(h2)), a[3]);
+        } else {
+            a[2] = cfma(c, cneg(pmr<PM>(h1)), a[2]);
+            a[3] = cfma(c, cneg(pmr<PM>(h2)), a[3]);
         }
     }
 }

@@ -576,7 +576,8 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
 
 // ---------------------------------------------------------------------------
 // Autotuner: time the inner (low-precision) dslash for every RHS blocking
-// (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,4}) and keep
+// (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,4}) and
+// streaming depth (t-slices per workgroup: automatic, 4, 8) and keep
 // the fastest per (volume, precision, nrhs, arch); optionally persisted as
 // text lines "V precision nrhs arch rhs_block us_per_rhs".
 // ---------------------------------------------------------------------------
@@ -654,12 +655,18 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     int rc = LMX_OK;
     best = 0;
     best_us = 1e30;
+    // t-slices per streaming workgroup: automatic, 4 or 8 (streaming lattices only)
+    const bool streams = ((long)c->g.L[0] * c->g.L[1] * c->g.L[2]) % 256 == 0;
+    const int tl_opts[3] = {0, 4, 8};
     for (int pi = 0; pi < 5 && rc == LMX_OK; ++pi) {
         const int per = per_opts[pi];
         if (per > nrhs) break;
+        for (int ti = 0; ti < 3 && rc == LMX_OK; ++ti) {
+        const int tl = tl_opts[ti];
+        if (tl > 0 && (!streams || c->g.L[3] % tl != 0)) continue;
         for (int nr = 1; nr <= 4 && rc == LMX_OK; ++nr) {
             if (nr == 3 || nr > per || per % nr != 0) continue;  // built: 1, 2, 4 per work item
-            const int blk = (per << 4) | nr;
+            const int blk = (tl << 12) | (per << 4) | nr;
             la.rhs_block = blk;
             for (int k = 0; k < 2 && rc == LMX_OK; ++k) rc = launch_wilson_lo(c, la);
             const int reps = 5;
@@ -674,6 +681,7 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
                 best_us = us;
                 best = blk;
             }
+        }
         }
     }
     (void)hipEventDestroy(e0);

@@ -59,10 +59,10 @@ int launch_wilson_lo(lmx_context* ctx, const LoArgs& a) {
         return launch_lo_sp<SpF32In, SpF32Out>(ctx, a, LinkF32{(const float2*)a.links, V},
                                                F32Clover{(const float2*)a.clover}, cl);
     if (a.link_prec == LMX_F16 && a.spinor_prec == LMX_F32)
-        return launch_lo_sp<SpF32In, SpF32Out>(ctx, a, LinkF16{(const __half2*)a.links, V},
+        return launch_lo_sp<SpF32In, SpF32Out>(ctx, a, LinkF16{(const int4*)a.links, V},
                                                F16Clover{(const __half2*)a.clover}, cl);
     if (a.link_prec == LMX_F16 && a.spinor_prec == LMX_H16)
-        return launch_lo_sp<SpH16In, SpH16Out>(ctx, a, LinkF16{(const __half2*)a.links, V},
+        return launch_lo_sp<SpH16In, SpH16Out>(ctx, a, LinkF16{(const int4*)a.links, V},
                                                F16Clover{(const __half2*)a.clover}, cl);
     set_error("unsupported precision pair (spinor %d, links %d)", a.spinor_prec, a.link_prec);
     return LMX_ERR_UNSUPPORTED;

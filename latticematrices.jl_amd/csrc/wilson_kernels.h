@@ -125,7 +125,8 @@ __device__ __forceinline__ void hop_half(cx<R> (&acc)[NR][12], const In& in, con
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         cx<R> psi[12], h[6], o[6];
-        in.ld12(rr[r], t, psi);
+        R c;
+        in.ld12s(rr[r], t, psi, c);
         if (GP) {
             if (w && !g.phase_one[MU]) {
                 if (FWD)
@@ -140,7 +141,7 @@ __device__ __forceinline__ void hop_half(cx<R> (&acc)[NR][12], const In& in, con
             matvec_fwd(u, h, o);
         else
             matvec_bwd(u, h, o);
-        reconstruct_add<PM, MU>(acc[r], o);
+        reconstruct_add_s<PM, MU>(acc[r], o, c);
     }
     pin_all(acc);
 }
@@ -155,6 +156,7 @@ struct EpiArgs {
 };
 
 enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
+constexpr int kMaxTL = 32;  // t-slices per workgroup at most (LDS of the fused reduction)
 enum { EPI_NONE = 0, EPI_PAP = 1 };
 
 // Finish one site from its own spinor psi: out = psi - kappa*hop (+ clover),
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
     using R = typename In::R;
     if (epi.st != nullptr && epi.st->done) return;
     if (epi.gate != nullptr && *epi.gate == 0) return;
-    __shared__ double lds4[4];
+    __shared__ double wpart[EPI == EPI_PAP ? NR : 1][EPI == EPI_PAP ? kMaxTL : 1][4];
     const int V3 = g.stride[3];
     const int nblk3 = V3 >> 8;
     const int ntc = g.L[3] / TL;
@@ -328,14 +330,15 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 cx<R> psi[12], h[6], o[6];
-                in.ld12(rr[r], tp, psi);
+                R c;
+                in.ld12s(rr[r], tp, psi, c);
                 if (GP) {
                     if (w && !g.phase_one[3]) phase_spinor(psi, g.ph_re[3], g.ph_im[3]);
                 }
                 half_project<FPM, 3>(psi, h);
                 if (!GP) neg_select(h, ng);
                 matvec_fwd(ut, h, o);
-                reconstruct_add<FPM, 3>(acc[r], o);
+                reconstruct_add_s<FPM, 3>(acc[r], o, c);
                 reconstruct_add<-FPM, 3>(acc[r], B[r]);
             }
         }
@@ -351,12 +354,28 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
             matvec_bwd(ut, h, B[r]);
         }
         if constexpr (EPI == EPI_PAP) {
-            const int blk = b3 + nblk3 * t;  // the 256 lexicographic sites this step covered
+            // wave sums of this step; the 4-wave combination waits for the end
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const double tot = block_tree(red[r], lds4);
-                if (r < nr && threadIdx.x == 0) epi.partials[rr[r] * epi.pstride + blk] = tot;
+                const double v = wave_halve(red[r]);
+                if ((threadIdx.x & 63) == 0) wpart[r][stp][threadIdx.x >> 6] = v;
             }
+        }
+    }
+    if constexpr (EPI == EPI_PAP) {
+        // one partial per step and RHS: the 256 lexicographic sites b3 + nblk3*t,
+        // combined in block_tree's order ((w0 + w2) + (w1 + w3))
+        __syncthreads();
+        const int tid = threadIdx.x;
+        if (tid < NR * TL) {
+            const int r = tid / TL, stp = tid - r * TL;
+            const double* w = wpart[r][stp];
+            const double tot = (w[0] + w[2]) + (w[1] + w[3]);
+            int rsel = rr[0];
+#pragma unroll
+            for (int q = 1; q < NR; ++q)
+                if (r == q) rsel = rr[q];
+            if (r < nr) epi.partials[(long)rsel * epi.pstride + b3 + nblk3 * (t0 + stp)] = tot;
         }
     }
 }
@@ -371,16 +390,19 @@ struct Shape {
 
 // t-streaming when the 3-D volume splits into 256-site blocks; TL and ZS keep
 // enough workgroups for the chip and keep one step's working set per XCD
-// small.  A context override (tune_tl / tune_zs) wins when it divides.
-inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks) {
+// small.  A context override (tune_tl / tune_zs) wins when it divides, then
+// the caller's (autotuned) hint.
+inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks, int tl_hint) {
     Shape sh{false, 1, 1};
     const Geom& g = ctx->g;
     const long V3 = (long)g.L[0] * g.L[1] * g.L[2];
     if (V3 % 256 != 0) return sh;
     const int nblk3 = (int)(V3 / 256);
     sh.stream = true;
-    if (ctx->tune_tl > 0 && g.L[3] % ctx->tune_tl == 0) {
+    if (ctx->tune_tl > 0 && ctx->tune_tl <= kMaxTL && g.L[3] % ctx->tune_tl == 0) {
         sh.TL = ctx->tune_tl;
+    } else if (tl_hint > 0 && tl_hint <= kMaxTL && g.L[3] % tl_hint == 0) {
+        sh.TL = tl_hint;
     } else {
         const int cand[5] = {16, 8, 4, 2, 1};
         for (int c : cand) {
@@ -410,10 +432,10 @@ inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks) {
 // (fused Re <p, Ap>) exists for D^dagger, the second half of D^dagger D.
 template <class In, class Lk, class Out, class Cl, class Aux, int NR, int EPI, bool SITE_NG, bool DONLY_OK>
 int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, double kappa, double ccoef, int ngroups, int nrhs,
-             int flags, bool clover, const EpiArgs& epi, int min_blocks) {
+             int flags, bool clover, const EpiArgs& epi, int min_blocks, int tl_hint = 0) {
     const bool dag = (flags & LMX_DAGGER) != 0;
     const bool donly = (flags & LMX_DONLY) != 0;
-    const Shape sh = pick_shape(ctx, ngroups, min_blocks);
+    const Shape sh = pick_shape(ctx, ngroups, min_blocks, tl_hint);
     const int nb_site = nblocks(ctx->V) * ngroups;
     const int nb_stream = (int)(ctx->V / 256 / sh.TL) * ngroups;
     const bool general = ctx->g.general != 0;
@@ -475,8 +497,9 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, double kappa
 template <class In, class Out, class Lk, class Cl, int NR>
 int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
     const double ccoef = (-a.kappa) * a.csw;
-    int per_launch = a.rhs_block >> 4;
+    int per_launch = (a.rhs_block >> 4) & 0xff;
     if (per_launch <= 0) per_launch = 1;
+    const int tl = (a.rhs_block >> 12) & 0xff;
     const int V = (int)ctx->V;
     for (int r0 = 0; r0 < a.nrhs; r0 += per_launch) {
         const int n = std::min(per_launch, a.nrhs - r0);
@@ -487,10 +510,10 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
         if (a.partials) {
             In x = In::at(a.aux, a.aux_norm, V, r0);
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, NR, EPI_PAP, false, false>(ctx, o, i, U, F, x, a.kappa, ccoef,
-                                                                               ngroups, n, a.flags, clover, epi, 512)));
+                                                                               ngroups, n, a.flags, clover, epi, 512, tl)));
         } else {
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, NR, EPI_NONE, false, false>(ctx, o, i, U, F, i, a.kappa, ccoef,
-                                                                                ngroups, n, a.flags, clover, epi, 512)));
+                                                                                ngroups, n, a.flags, clover, epi, 512, tl)));
         }
     }
     return LMX_OK;
