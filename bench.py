@@ -75,6 +75,23 @@ def timed(lat, fn, reps):
     return s.elapsed_time(e) / reps
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stream_fp64_dw_32x32x32x32.json")
+
+
+def measured_traffic(L):
+    """HBM bytes per FP64 D_W launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated by a 1 GiB copy); None if absent or for
+    another volume."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if tuple(L) != (32, 32, 32, 32):
+        return None, None
+    return float(t["traffic_bytes_per_launch"]), "profiles/k_stream_fp64_dw_32x32x32x32.md"
+
+
 def lo_dslash_bytes(precision, n):
     """Algorithmic bytes per site per RHS of the low-precision dslash: the
     links once per launch of n RHS, the input and output spinor once each."""
@@ -201,6 +218,7 @@ def main():
     log(f"bench: mixed {args.precision} x{n}: {tm:.3f} s/batch = {per_rhs:.4f} s/RHS; speedup vs FP64 "
         f"{t64 / per_rhs:.2f}x; max true rel residual {max(true_res):.3e}; RHS0 error vs FP64 {err0:.3e}")
 
+    traffic, traffic_src = measured_traffic(L)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(U, b[0], L, args.kappa, args.cpu_iters, res64.iterations)
@@ -237,9 +255,11 @@ def main():
                          "algorithmic_bytes_per_site_per_rhs": bytes_dm, "achieved_gbs": ach_dm,
                          "frac": ach_dm / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "achieved": ach_d64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_d64 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stream FP64 D_W (32^4)", "bytes_per_launch": bytes_d64,
-                     "us_per_launch": t_d64 * 1e3},
+                     "frac": ach_d64 / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_stream<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, -1, 0, 1, 0, false> "
+                               "(FP64 D_W, 32^4)",
+                     "bytes_per_launch": bytes_d64, "us_per_launch": t_d64 * 1e3,
+                     "traffic_source": traffic_src},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
