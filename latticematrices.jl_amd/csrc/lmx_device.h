@@ -303,6 +303,9 @@ struct SpH16Out {
 // Links: FP64 / FP32 as double2 / float2 [4][9][V]; FP16 packed per
 // direction as two 16-byte words (entries 0-3, 4-7) and one 4-byte word
 // (entry 8): int4 [4][2][V] followed by half2 [4][V], three loads a link.
+#ifndef LMX_NT_LINKS
+#define LMX_NT_LINKS 0
+#endif
 struct LinkF64 {
     const double2* __restrict__ p;
     int V;
@@ -311,8 +314,14 @@ struct LinkF64 {
         const double2* q = p + (size_t)mu * 9 * V;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
+#if LMX_NT_LINKS
+            const lmx_dv2* a = (const lmx_dv2*)((const char*)(q + (size_t)k * V) + (unsigned)s * 16u);
+            lmx_dv2 w = __builtin_nontemporal_load(a);
+            u[k] = mk<R>(R(w.x), R(w.y));
+#else
             double2 v = ldv(q + (size_t)k * V, s);
             u[k] = mk<R>(R(v.x), R(v.y));
+#endif
         }
     }
 };
@@ -354,11 +363,18 @@ struct LinkF16 {
 // that "modifies" the values stops the scheduler from sinking the arithmetic
 // that produced them below the loads that follow (which otherwise keeps
 // every loaded neighbour live at once and spills).
+#ifndef LMX_PIN_MEM
+#define LMX_PIN_MEM 1
+#endif
 template <typename R>
 __device__ __forceinline__ void pin_acc(cx<R> (&a)[12]) {
 #pragma unroll
     for (int k = 0; k < 12; k += 2)
+#if LMX_PIN_MEM
         asm volatile("" : "+v"(a[k].re), "+v"(a[k].im), "+v"(a[k + 1].re), "+v"(a[k + 1].im)::"memory");
+#else
+        asm volatile("" : "+v"(a[k].re), "+v"(a[k].im), "+v"(a[k + 1].re), "+v"(a[k + 1].im));
+#endif
 }
 
 // ---------------------------------------------------------------------------

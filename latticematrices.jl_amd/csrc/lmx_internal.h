@@ -107,9 +107,11 @@ int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover,
                       double2* out, const double2* in, int nrhs, int flags, const Epilogue& epi);
 
 // Low-precision apply: spinors LMX_F32 or LMX_H16 (data + per-site scale),
-// links LMX_F32 or LMX_F16, FP32 arithmetic.  With partials set, the launch
-// also stores block partials of Re <aux, out> per RHS (row r at
-// partials + r * pstride, one per 256 consecutive sites).
+// links LMX_F32 or LMX_F16, FP32 arithmetic.  epi selects a fused epilogue
+// whose block partials (one per 256 consecutive sites) go to row r of
+// partials (partials + r * pstride): 2 = |out|^2 on D; 3 = on D^dagger,
+// out = Ap is not stored but r -= alpha Ap, x += alpha p (p = aux; r in the
+// spinor format, x FP32) with |r|^2, skipping RHS whose done flag is set.
 struct LoArgs {
     int link_prec, spinor_prec;
     const void* links;
@@ -125,6 +127,14 @@ struct LoArgs {
     long pstride;
     int nrhs, flags, rhs_block;
     const int* gate;
+    int epi;
+    void* r;
+    float* r_norm;
+    float2* x;
+    const double* alpha;
+    int astride;
+    const int* done;
+    int dstride;
 };
 int launch_wilson_lo(lmx_context* ctx, const LoArgs& a);
 

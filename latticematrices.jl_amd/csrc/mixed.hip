@@ -3,8 +3,10 @@
 //
 // Mixed solver (per right-hand side i, all RHS advanced in lockstep by
 // batched kernels; every scalar lives on the device):
-//   inner CG in FP32 spinors with FP32 or FP16-stored links (FP32 math),
-//   p.Ap and |r|^2 accumulated in FP64, alpha/beta per RHS on the device;
+//   inner CG in FP32 arithmetic on FP32 or 16-bit stored vectors with FP32
+//   or FP16 links; p.Ap = |D p|^2 and |r|^2 accumulated in FP64 in the
+//   stencils' epilogues (the x/r update rides on the D^dagger launch),
+//   alpha/beta per RHS on the device;
 //   a reliable update replaces the inner residual by the FP64 true residual
 //   r = b - A x (A = D^dagger D in FP64) and folds the inner solution into the
 //   FP64 solution x, when |r| drops below delta * (largest |r| since the
@@ -17,6 +19,7 @@
 // on DdagDOp (src/Operators/DiracOperators.jl:104-121).
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
@@ -45,7 +48,7 @@ struct InnerVec {
 
 struct MixPtrs {
     float2* xlo;
-    InnerVec r, p, Ap, t;
+    InnerVec r, p, t;
     double2* r64;
     double2* t64;
     double* partials;
@@ -80,7 +83,6 @@ static size_t layout(lmx_context* ctx, int nrhs, int sp, MixPtrs* P, char* base)
     q.xlo = (float2*)take(f32);
     q.r = inner();
     q.p = inner();
-    q.Ap = inner();
     q.t = inner();
     q.r64 = (double2*)take(f64);
     q.t64 = (double2*)take(f64);
@@ -143,7 +145,7 @@ __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ parti
             retire(st, batch, LMX_CG_BREAKDOWN, 0);
         else if (st->rtrue <= st->tol)
             retire(st, batch, LMX_CG_CONVERGED, 1);
-    } else if (KIND == MX_PAP) {  // A = Re <p, Ap>
+    } else if (KIND == MX_PAP) {  // A = p.Ap = |D p|^2
         st->pAp = A;
         if (!isfinite(A))
             retire(st, batch, LMX_CG_BREAKDOWN, 0);
@@ -222,37 +224,6 @@ __global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta,
         batch->delta = delta;
         batch->rmax = rmax;
     }
-}
-
-// x_lo += alpha p; r_lo -= alpha Ap; |r_lo|^2 (of the unrounded update)
-template <class SI, class SO>
-__global__ void __launch_bounds__(256) k_mx_update_xr(float2* __restrict__ xlo, SO rout, SI rin, SI p, SI Ap, int V,
-                                                      double* partials, const MixRhs* rhs) {
-    const int i = blockIdx.y;
-    if (rhs[i].done) return;
-    const float a = (float)rhs[i].alpha;
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    double v = 0.0;
-    if (s < V) {
-        const size_t o = (size_t)i * 12 * V;
-        cx<float> pv[12], av[12], rv[12];
-        p.ld12(i, s, pv);
-        Ap.ld12(i, s, av);
-        rin.ld12(i, s, rv);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            const size_t e = o + (size_t)k * V + s;
-            float2 xv = xlo[e];
-            xv.x = fmaf(a, pv[k].re, xv.x);
-            xv.y = fmaf(a, pv[k].im, xv.y);
-            xlo[e] = xv;
-            rv[k].re = fmaf(-a, av[k].re, rv[k].re);
-            rv[k].im = fmaf(-a, av[k].im, rv[k].im);
-            v += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
-        }
-        rout.st12(i, s, rv);
-    }
-    row_partials(v, 0.0, partials);
 }
 
 // reliable update, part 1: x += x_lo (FP64), x_lo = 0
@@ -358,8 +329,13 @@ static LoArgs lo_args(lmx_wilson* op, int link_prec, int spinor_prec) {
     return a;
 }
 
-// t = D p; Ap = D^dagger t with the block partials of Re <p, Ap> fused in.
-static int apply_ddagd_lo(MixRun& R) {
+// One inner iteration's operator work, in two launches with fused epilogues:
+//   t = D p, with the block partials of |t|^2 = p.Ap (FP64 sums), then
+//   (after the finalize has set alpha) Ap = D^dagger t, kept in registers:
+//   r -= alpha Ap, x_inner += alpha p, with the block partials of |r|^2.
+static_assert(sizeof(MixRhs) % sizeof(double) == 0, "MixRhs is addressed in doubles and ints");
+
+static int apply_d_norm(MixRun& R) {
     LoArgs a = lo_args(R.op, R.link_prec, R.spinor_prec);
     a.nrhs = R.nrhs;
     a.rhs_block = R.rhs_block;
@@ -369,25 +345,36 @@ static int apply_ddagd_lo(MixRun& R) {
     a.in = R.w.p.p;
     a.in_norm = R.w.p.n;
     a.flags = 0;
-    LMX_TRY(launch_wilson_lo(R.ctx, a));
-    a.out = R.w.Ap.p;
-    a.out_norm = R.w.Ap.n;
+    a.epi = 2;
+    a.partials = R.w.partials;
+    a.pstride = 2L * nblocks(R.ctx->V);
+    return launch_wilson_lo(R.ctx, a);
+}
+
+static int apply_ddag_xr(MixRun& R) {
+    LoArgs a = lo_args(R.op, R.link_prec, R.spinor_prec);
+    a.nrhs = R.nrhs;
+    a.rhs_block = R.rhs_block;
+    a.gate = &R.w.batch->n_active;
+    a.out = nullptr;  // Ap is consumed in registers
     a.in = R.w.t.p;
     a.in_norm = R.w.t.n;
     a.aux = R.w.p.p;
     a.aux_norm = R.w.p.n;
+    a.flags = LMX_DAGGER;
+    a.epi = 3;
+    a.r = R.w.r.p;
+    a.r_norm = R.w.r.n;
+    a.x = R.w.xlo;
+    a.alpha = (const double*)((const char*)R.w.rhs + offsetof(MixRhs, alpha));
+    a.astride = (int)(sizeof(MixRhs) / sizeof(double));
+    a.done = (const int*)((const char*)R.w.rhs + offsetof(MixRhs, done));
+    a.dstride = (int)(sizeof(MixRhs) / sizeof(int));
     a.partials = R.w.partials;
     a.pstride = 2L * nblocks(R.ctx->V);
-    a.flags = LMX_DAGGER;
     return launch_wilson_lo(R.ctx, a);
 }
 
-template <class SI, class SO>
-static void enqueue_blas_xr(MixRun& R, dim3 grid) {
-    const int V = (int)R.ctx->V;
-    hipLaunchKernelGGL((k_mx_update_xr<SI, SO>), grid, dim3(256), 0, R.ctx->stream, R.w.xlo, out_of<SO>(R.w.r, V),
-                       in_of<SI>(R.w.r, V), in_of<SI>(R.w.p, V), in_of<SI>(R.w.Ap, V), V, R.w.partials, R.w.rhs);
-}
 template <class SO>
 static void enqueue_true_res(MixRun& R, dim3 grid, int force) {
     const int V = (int)R.ctx->V;
@@ -412,15 +399,12 @@ static int enqueue_iteration(MixRun& R) {
     const int V = (int)c->V;
     const bool h16 = R.spinor_prec == LMX_H16;
     dim3 grid(nblocks(c->V), R.nrhs), blk(256);
-    LMX_TRY(apply_ddagd_lo(R));
     const int nb = nblocks(c->V);
     MixRhs* rhs = R.w.rhs;
     MixBatch* bt = R.w.batch;
+    LMX_TRY(apply_d_norm(R));
     hipLaunchKernelGGL(k_mx_fin<MX_PAP>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
-    if (h16)
-        enqueue_blas_xr<SpH16In, SpH16Out>(R, grid);
-    else
-        enqueue_blas_xr<SpF32In, SpF32Out>(R, grid);
+    LMX_TRY(apply_ddag_xr(R));
     hipLaunchKernelGGL(k_mx_fin<MX_RR>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
     hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, R.x, R.w.xlo, V, rhs, bt, 0);
     LMX_TRY(apply_ddagd64(R, R.w.r64, R.x, R.w.t64, &bt->do_update));

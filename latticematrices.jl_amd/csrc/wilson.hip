@@ -18,16 +18,17 @@ int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover,
     SpF64In i{in, V};
     LinkF64 l{U, V};
     F64Clover F{clover};
-    EpiArgs epi{e.partials, 0, e.st, e.gate, nullptr, 0};
+    EpiArgs epi{e.partials, 0, e.st, e.gate};
+    const NoEx none{};
     if (e.kind == 1) {
         LMX_CHECK(nrhs == 1, LMX_ERR_UNSUPPORTED, "fused p.Ap needs a single right-hand side");
         SpF64In aux{e.aux, V};
-        LMX_TRY((dispatch<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, 1, EPI_PAP, true, true>(
-            ctx, o, i, l, F, aux, kappa, ccoef, 1, 1, flags, clover != nullptr, epi, 256)));
+        LMX_TRY((dispatch<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, NoEx, 1, EPI_PAP, true, true>(
+            ctx, o, i, l, F, aux, none, kappa, ccoef, 1, 1, flags, clover != nullptr, epi, 256)));
         return launch_finalize(ctx, nblocks(ctx->V), FIN_PAP, false, e.st, nullptr);
     }
-    return dispatch<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, 1, EPI_NONE, true, true>(
-        ctx, o, i, l, F, i, kappa, ccoef, nrhs, nrhs, flags, clover != nullptr, epi, 256);
+    return dispatch<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, NoEx, 1, EPI_NONE, true, true>(
+        ctx, o, i, l, F, i, none, kappa, ccoef, nrhs, nrhs, flags, clover != nullptr, epi, 256);
 }
 
 // ---------------------------------------------------------------------------

@@ -147,24 +147,45 @@ __device__ __forceinline__ void hop_half(cx<R> (&acc)[NR][12], const In& in, con
 }
 
 struct EpiArgs {
-    double* partials;  // block partials of Re <aux, out>, row r at partials + r * pstride
+    double* partials;  // block partials of the fused reduction, row r at partials + r * pstride
     long pstride;
     CGState* st;       // early exit when st->done
     const int* gate;   // early exit while *gate == 0
-    const int* rdone;  // per-RHS retirement flags (stride rdone_stride ints), or null
-    int rdone_stride;
 };
 
 enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
 constexpr int kMaxTL = 32;  // t-slices per workgroup at most (LDS of the fused reduction)
-enum { EPI_NONE = 0, EPI_PAP = 1 };
+#ifndef LMX_CARRY_NEXT
+#define LMX_CARRY_NEXT 1
+#endif
+
+// Fused epilogues: EPI_PAP   Re <aux, out> (the CG's p.Ap on D^dagger, FP64 CG);
+//                  EPI_NORM  |out|^2 (p.Ap = |D p|^2 on D, mixed solver);
+//                  EPI_XR    out = Ap is not stored: r -= alpha Ap, x += alpha p
+//                            (p = aux) and |r|^2 (mixed solver, on D^dagger).
+enum { EPI_NONE = 0, EPI_PAP = 1, EPI_NORM = 2, EPI_XR = 3 };
+
+struct NoEx {};
+// Extra fields of EPI_XR, all offset to the launch's first RHS.
+template <class In, class Out>
+struct XrEx {
+    In rin;        // r (read) ...
+    Out rout;      // ... and written in place
+    SpF32In xin;   // inner solution x (FP32)
+    SpF32Out xout;
+    const double* alpha;  // alpha of RHS r at alpha[r * astride]
+    int astride;
+    const int* done;      // retired RHS (no update) at done[r * dstride]
+    int dstride;
+};
 
 // Finish one site from its own spinor psi: out = psi - kappa*hop (+ clover),
-// or 1/2 hop; optional Re <aux, out> for the fused CG reduction.
-template <int MODE, int EPI, class Out, class Cl, class Aux, typename R>
+// or 1/2 hop; then the fused epilogue, returning the site's contribution to
+// its reduction.
+template <int MODE, int EPI, class Out, class Cl, class Aux, class Ex, typename R>
 __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)[12], const Cl& F, const Aux& aux,
-                                              int r, int s, const Geom& g, R kappa, R ccoef, const cx<R> (&acc)[12],
-                                              bool store) {
+                                              const Ex& ex, int r, int s, const Geom& g, R kappa, R ccoef,
+                                              const cx<R> (&acc)[12], bool store) {
     cx<R> res[12];
     if (MODE == MODE_DONLY) {
         const R half = R(1) / R(2);
@@ -179,8 +200,27 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
             for (int c = 0; c < 3; ++c) res[sp * 3 + c] = cadd(psi[sp * 3 + c], rmul(-kappa, acc[c * 4 + sp]));
         if constexpr (MODE == MODE_CLOVER) clover_add<R>(F, s, g.V, psi, ccoef, res);
     }
-    if (store) out.st12(r, s, res);
     double red = 0.0;
+    if constexpr (EPI == EPI_XR) {
+        const R a = R(ex.alpha[(long)r * ex.astride]);
+        cx<R> rv[12], pv[12], xv[12];
+        ex.rin.ld12(r, s, rv);
+        aux.ld12(r, s, pv);
+        ex.xin.ld12(r, s, xv);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            rv[k] = mk<R>(fmaR(-a, res[k].re, rv[k].re), fmaR(-a, res[k].im, rv[k].im));
+            xv[k] = mk<R>(fmaR(a, pv[k].re, xv[k].re), fmaR(a, pv[k].im, xv[k].im));
+            red += double(rv[k].re) * double(rv[k].re) + double(rv[k].im) * double(rv[k].im);
+        }
+        if (store && ex.done[(long)r * ex.dstride] == 0) {
+            ex.rout.st12(r, s, rv);
+            ex.xout.st12(r, s, xv);
+        }
+        return red;
+    } else {
+        if (store) out.st12(r, s, res);
+    }
     if constexpr (EPI == EPI_PAP) {
         // Re sum conj(p) * Ap over the site, in kernel_dot_D order (spin, colour)
         cx<R> pv[12];
@@ -191,6 +231,11 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
                                 mk<double>(double(res[k].re), double(res[k].im)));
             red = red + t.re;
         }
+    }
+    if constexpr (EPI == EPI_NORM) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            red += double(res[k].re) * double(res[k].re) + double(res[k].im) * double(res[k].im);
     }
     return red;
 }
@@ -206,8 +251,8 @@ __device__ __forceinline__ void rhs_rows(int rg, int nrhs, int (&rr)[NR], int& n
 // ---------------------------------------------------------------------------
 // Per-site kernel
 // ---------------------------------------------------------------------------
-template <class In, class Lk, class Out, class Cl, class Aux, int FPM, int MODE, int NR, int EPI, bool GP>
-__global__ void __launch_bounds__(256) k_site(Out out, In in, Lk U, Cl F, Aux aux, Geom g, double kappa_d,
+template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int FPM, int MODE, int NR, int EPI, bool GP>
+__global__ void __launch_bounds__(256) k_site(Out out, In in, Lk U, Cl F, Aux aux, Ex ex, Geom g, double kappa_d,
                                               double ccoef_d, int ngroups, int nrhs, EpiArgs epi) {
     using R = typename In::R;
     if (epi.st != nullptr && epi.st->done) return;
@@ -241,10 +286,11 @@ __global__ void __launch_bounds__(256) k_site(Out out, In in, Lk U, Cl F, Aux au
         for (int r = 0; r < NR; ++r) {
             cx<R> psi[12];
             if (MODE != MODE_DONLY) in.ld12(rr[r], s, psi);
-            red[r] = finish_site<MODE, EPI>(out, psi, F, aux, rr[r], s, g, R(kappa_d), R(ccoef_d), acc[r], r < nr);
+            red[r] = finish_site<MODE, EPI>(out, psi, F, aux, ex, rr[r], s, g, R(kappa_d), R(ccoef_d), acc[r],
+                                            r < nr);
         }
     }
-    if constexpr (EPI == EPI_PAP) {
+    if constexpr (EPI != EPI_NONE) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             const double tot = block_tree(red[r], lds4);
@@ -256,13 +302,13 @@ __global__ void __launch_bounds__(256) k_site(Out out, In in, Lk U, Cl F, Aux au
 // ---------------------------------------------------------------------------
 // t-streaming kernel (3-D volume a multiple of 256)
 // ---------------------------------------------------------------------------
-template <class In, class Lk, class Out, class Cl, class Aux, int FPM, int MODE, int NR, int EPI, bool GP>
-__global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux aux, Geom g, double kappa_d,
+template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int FPM, int MODE, int NR, int EPI, bool GP>
+__global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux aux, Ex ex, Geom g, double kappa_d,
                                                 double ccoef_d, int ngroups, int nrhs, int TL, int ZS, EpiArgs epi) {
     using R = typename In::R;
     if (epi.st != nullptr && epi.st->done) return;
     if (epi.gate != nullptr && *epi.gate == 0) return;
-    __shared__ double wpart[EPI == EPI_PAP ? NR : 1][EPI == EPI_PAP ? kMaxTL : 1][4];
+    __shared__ double wpart[EPI != EPI_NONE ? NR : 1][EPI != EPI_NONE ? kMaxTL : 1][4];
     const int V3 = g.stride[3];
     const int nblk3 = V3 >> 8;
     const int ntc = g.L[3] / TL;
@@ -305,10 +351,26 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
             matvec_bwd(u, h, B[r]);
         }
     }
+    // FP64 (memory-bound): the t+1 spinor loaded for the forward t-hop is
+    // carried in registers as the next step's own spinor instead of being
+    // re-read (by then it has usually left L2).  The low-precision kernels
+    // are instruction-bound and keep their registers for occupancy.
+    constexpr bool kCarry = LMX_CARRY_NEXT && sizeof(R) == 8;
+    constexpr int NC = kCarry ? NR : 1;
+    cx<R> cur[NC][12], nxt[NC][12];
+    R curc[NC], nxtc[NC];
     for (int stp = 0; stp < TL; ++stp) {
         const int t = t0 + stp;
         x[3] = t;
         const int s = s3 + V3 * t;
+        if constexpr (kCarry) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+#pragma unroll
+                for (int k = 0; k < 12; ++k) cur[r][k] = nxt[r][k];
+                curc[r] = nxtc[r];
+            }
+        }
         cx<R> acc[NR][12];
 #pragma unroll
         for (int r = 0; r < NR; ++r)
@@ -332,6 +394,11 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
                 cx<R> psi[12], h[6], o[6];
                 R c;
                 in.ld12s(rr[r], tp, psi, c);
+                if constexpr (kCarry) {  // slice t+1 is the next step's own slice: keep it
+#pragma unroll
+                    for (int k = 0; k < 12; ++k) nxt[r][k] = psi[k];
+                    nxtc[r] = c;
+                }
                 if (GP) {
                     if (w && !g.phase_one[3]) phase_spinor(psi, g.ph_re[3], g.ph_im[3]);
                 }
@@ -348,12 +415,17 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
             // this slice's own spinor: the output's diagonal term and the next
             // slice's backward t-hop
             cx<R> p[12], h[6];
-            in.ld12(rr[r], s, p);
-            red[r] = finish_site<MODE, EPI>(out, p, F, aux, rr[r], s, g, kappa, ccoef, acc[r], r < nr);
+            if (kCarry && stp > 0) {
+#pragma unroll
+                for (int k = 0; k < 12; ++k) p[k] = mk<R>(cur[r][k].re * curc[r], cur[r][k].im * curc[r]);
+            } else {
+                in.ld12(rr[r], s, p);
+            }
+            red[r] = finish_site<MODE, EPI>(out, p, F, aux, ex, rr[r], s, g, kappa, ccoef, acc[r], r < nr);
             half_project<-FPM, 3>(p, h);
             matvec_bwd(ut, h, B[r]);
         }
-        if constexpr (EPI == EPI_PAP) {
+        if constexpr (EPI != EPI_NONE) {
             // wave sums of this step; the 4-wave combination waits for the end
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
@@ -362,7 +434,7 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
             }
         }
     }
-    if constexpr (EPI == EPI_PAP) {
+    if constexpr (EPI != EPI_NONE) {
         // one partial per step and RHS: the 256 lexicographic sites b3 + nblk3*t,
         // combined in block_tree's order ((w0 + w2) + (w1 + w3))
         __syncthreads();
@@ -428,11 +500,12 @@ inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks, int
 
 // GP (complex boundary phases) runs in the per-site kernel only; SITE_NG
 // instantiates the per-site kernel for +-1 phases too (else the GP kernel
-// covers lattices whose 3-D volume is not a multiple of 256).  EPI_PAP
-// (fused Re <p, Ap>) exists for D^dagger, the second half of D^dagger D.
-template <class In, class Lk, class Out, class Cl, class Aux, int NR, int EPI, bool SITE_NG, bool DONLY_OK>
-int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, double kappa, double ccoef, int ngroups, int nrhs,
-             int flags, bool clover, const EpiArgs& epi, int min_blocks, int tl_hint = 0) {
+// covers lattices whose 3-D volume is not a multiple of 256).  The fused
+// epilogues exist where D^dagger D uses them: EPI_PAP and EPI_XR on
+// D^dagger, EPI_NORM on D.
+template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int NR, int EPI, bool SITE_NG, bool DONLY_OK>
+int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex, double kappa, double ccoef,
+             int ngroups, int nrhs, int flags, bool clover, const EpiArgs& epi, int min_blocks, int tl_hint = 0) {
     const bool dag = (flags & LMX_DAGGER) != 0;
     const bool donly = (flags & LMX_DONLY) != 0;
     const Shape sh = pick_shape(ctx, ngroups, min_blocks, tl_hint);
@@ -440,32 +513,38 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, double kappa
     const int nb_stream = (int)(ctx->V / 256 / sh.TL) * ngroups;
     const bool general = ctx->g.general != 0;
     hipStream_t st = ctx->stream;
-    if (EPI == EPI_PAP && (!dag || donly)) {
-        set_error("the fused p.Ap reduction exists for D^dagger only");
+    if (EPI != EPI_NONE && (donly || dag != (EPI != EPI_NORM))) {
+        set_error("fused reduction %d not built for this operator (p.Ap and x/r updates on D^dagger, |D p|^2 on D)",
+                  EPI);
         return LMX_ERR_UNSUPPORTED;
     }
     if (donly && !DONLY_OK) {
         set_error("hopping-only operator is built in FP64 only");
         return LMX_ERR_UNSUPPORTED;
     }
-#define LMX_GO(FPM, MODE)                                                                                             \
-    do {                                                                                                              \
-        if (general || (!sh.stream && !SITE_NG))                                                                      \
-            hipLaunchKernelGGL((k_site<In, Lk, Out, Cl, Aux, FPM, MODE, NR, EPI, true>), dim3(nb_site), dim3(256), 0, \
-                               st, out, in, U, F, aux, ctx->g, kappa, ccoef, ngroups, nrhs, epi);                     \
-        else if (sh.stream)                                                                                           \
-            hipLaunchKernelGGL((k_stream<In, Lk, Out, Cl, Aux, FPM, MODE, NR, EPI, false>), dim3(nb_stream),         \
-                               dim3(256), 0, st, out, in, U, F, aux, ctx->g, kappa, ccoef, ngroups, nrhs, sh.TL,     \
-                               sh.ZS, epi);                                                                           \
-        else if constexpr (SITE_NG)                                                                                   \
-            hipLaunchKernelGGL((k_site<In, Lk, Out, Cl, Aux, FPM, MODE, NR, EPI, false>), dim3(nb_site), dim3(256),  \
-                               0, st, out, in, U, F, aux, ctx->g, kappa, ccoef, ngroups, nrhs, epi);                  \
+#define LMX_GO(FPM, MODE)                                                                                      \
+    do {                                                                                                       \
+        if (general || (!sh.stream && !SITE_NG))                                                               \
+            hipLaunchKernelGGL((k_site<In, Lk, Out, Cl, Aux, Ex, FPM, MODE, NR, EPI, true>), dim3(nb_site),    \
+                               dim3(256), 0, st, out, in, U, F, aux, ex, ctx->g, kappa, ccoef, ngroups, nrhs, epi); \
+        else if (sh.stream)                                                                                    \
+            hipLaunchKernelGGL((k_stream<In, Lk, Out, Cl, Aux, Ex, FPM, MODE, NR, EPI, false>), dim3(nb_stream), \
+                               dim3(256), 0, st, out, in, U, F, aux, ex, ctx->g, kappa, ccoef, ngroups, nrhs,  \
+                               sh.TL, sh.ZS, epi);                                                             \
+        else if constexpr (SITE_NG)                                                                            \
+            hipLaunchKernelGGL((k_site<In, Lk, Out, Cl, Aux, Ex, FPM, MODE, NR, EPI, false>), dim3(nb_site),   \
+                               dim3(256), 0, st, out, in, U, F, aux, ex, ctx->g, kappa, ccoef, ngroups, nrhs, epi); \
     } while (0)
-    if constexpr (EPI == EPI_PAP) {
+    if constexpr (EPI == EPI_PAP || EPI == EPI_XR) {
         if (clover)
             LMX_GO(1, MODE_CLOVER);
         else
             LMX_GO(1, MODE_WILSON);
+    } else if constexpr (EPI == EPI_NORM) {
+        if (clover)
+            LMX_GO(-1, MODE_CLOVER);
+        else
+            LMX_GO(-1, MODE_WILSON);
     } else {
         if (donly) {
             if constexpr (DONLY_OK) {
@@ -491,9 +570,10 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, double kappa
     return LMX_OK;
 }
 
-// One low-precision (In, Out, Lk, Cl, NR) combination: rhs_block >> 4
-// right-hand sides per launch, NR per work item; fused Re <aux, out> when
-// a.partials is set.  Instantiated per combination in wilson_lo.hip.
+// One low-precision (In, Out, Lk, Cl, NR) combination: (rhs_block >> 4) & 0xff
+// right-hand sides per launch, NR per work item, (rhs_block >> 12) t-slices
+// per streaming workgroup; a.epi selects the fused epilogue.  Instantiated
+// per combination in wilson_lo.hip.
 template <class In, class Out, class Lk, class Cl, int NR>
 int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
     const double ccoef = (-a.kappa) * a.csw;
@@ -506,14 +586,24 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
         const int ngroups = (n + NR - 1) / NR;
         Out o = Out::at(a.out, a.out_norm, V, r0);
         In i = In::at(a.in, a.in_norm, V, r0);
-        EpiArgs epi{a.partials ? a.partials + (long)r0 * a.pstride : nullptr, a.pstride, nullptr, a.gate, nullptr, 0};
-        if (a.partials) {
-            In x = In::at(a.aux, a.aux_norm, V, r0);
-            LMX_TRY((dispatch<In, Lk, Out, Cl, In, NR, EPI_PAP, false, false>(ctx, o, i, U, F, x, a.kappa, ccoef,
-                                                                               ngroups, n, a.flags, clover, epi, 512, tl)));
+        EpiArgs epi{a.partials ? a.partials + (long)r0 * a.pstride : nullptr, a.pstride, nullptr, a.gate};
+        const NoEx none{};
+        if (a.epi == EPI_XR) {
+            In p = In::at(a.aux, a.aux_norm, V, r0);
+            XrEx<In, Out> ex{In::at(a.r, a.r_norm, V, r0), Out::at(a.r, a.r_norm, V, r0),
+                             SpF32In::at(a.x, nullptr, V, r0), SpF32Out::at(a.x, nullptr, V, r0),
+                             a.alpha + (long)r0 * a.astride, a.astride, a.done + (long)r0 * a.dstride, a.dstride};
+            LMX_TRY((dispatch<In, Lk, Out, Cl, In, XrEx<In, Out>, NR, EPI_XR, false, false>(
+                ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl)));
+        } else if (a.epi == EPI_NORM) {
+            LMX_TRY((dispatch<In, Lk, Out, Cl, In, NoEx, NR, EPI_NORM, false, false>(
+                ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl)));
+        } else if (a.epi == EPI_NONE) {
+            LMX_TRY((dispatch<In, Lk, Out, Cl, In, NoEx, NR, EPI_NONE, false, false>(
+                ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl)));
         } else {
-            LMX_TRY((dispatch<In, Lk, Out, Cl, In, NR, EPI_NONE, false, false>(ctx, o, i, U, F, i, a.kappa, ccoef,
-                                                                                ngroups, n, a.flags, clover, epi, 512, tl)));
+            set_error("fused epilogue %d is not built for low-precision spinors", a.epi);
+            return LMX_ERR_UNSUPPORTED;
         }
     }
     return LMX_OK;
