@@ -309,6 +309,17 @@ struct SpH16Out {
 struct LinkF64 {
     const double2* __restrict__ p;
     int V;
+    // nontemporal: a link used once, whose lines should not displace cached spinors
+    template <typename R>
+    __device__ __forceinline__ void ld9_nt(int mu, int s, cx<R>* u) const {
+        const double2* q = p + (size_t)mu * 9 * V;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const lmx_dv2* a = (const lmx_dv2*)((const char*)(q + (size_t)k * V) + (unsigned)s * 16u);
+            lmx_dv2 w = __builtin_nontemporal_load(a);
+            u[k] = mk<R>(R(w.x), R(w.y));
+        }
+    }
     template <typename R>
     __device__ __forceinline__ void ld9(int mu, int s, cx<R>* u) const {
         const double2* q = p + (size_t)mu * 9 * V;

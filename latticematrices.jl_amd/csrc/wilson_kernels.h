@@ -24,6 +24,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 
 #include "lmx_internal.h"
 
@@ -157,6 +158,12 @@ enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
 constexpr int kMaxTL = 32;  // t-slices per workgroup at most (LDS of the fused reduction)
 #ifndef LMX_CARRY_NEXT
 #define LMX_CARRY_NEXT 1
+#endif
+#ifndef LMX_NT_UT
+#define LMX_NT_UT 1
+#endif
+#ifndef LMX_TWO_LANE
+#define LMX_TWO_LANE 0
 #endif
 
 // Fused epilogues: EPI_PAP   Re <aux, out> (the CG's p.Ap on D^dagger, FP64 CG);
@@ -388,7 +395,12 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
             bool w;
             const int tp = nbr_fwd(s, x, 3, g, w);
             const bool ng = w && g.phase_neg[3];
-            U.template ld9<R>(3, s, ut);
+#if LMX_NT_UT
+            if constexpr (std::is_same<Lk, LinkF64>::value)
+                U.template ld9_nt<R>(3, s, ut);  // U_t(x) is used once: its backward use is the carried B
+            else
+#endif
+                U.template ld9<R>(3, s, ut);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 cx<R> psi[12], h[6], o[6];
@@ -448,6 +460,264 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
             for (int q = 1; q < NR; ++q)
                 if (r == q) rsel = rr[q];
             if (r < nr) epi.partials[(long)rsel * epi.pstride + b3 + nblk3 * (t0 + stp)] = tot;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 t-streaming kernel with two lanes per site (512-thread workgroups over
+// the same 256-site blocks).  Lane j of a pair (j = thread & 1) computes
+// half-spinor component j of every projection: it loads spin j and the
+// partner spin of the direction (3 - j for mu = x, y; 2 + j for mu = z, t)
+// of each neighbour, applies the link to 3 colours, and accumulates spin j
+// plus one of spins 2 / 3: spin 3 - j for mu = x, y and spin 2 + j for
+// mu = z, t, so the pair swaps those two partial sums once, after mu = y.
+// Every accumulator still receives its eight terms in the reference order
+// (mu = x fwd, x bwd, ..., t fwd, t bwd) with the same operations, so the
+// result is bit-identical to k_stream / k_site; the gain is twice the waves
+// with fewer registers each for the latency-bound FP64 stencil.
+// ---------------------------------------------------------------------------
+template <int MU, int PM, typename R>
+__device__ __forceinline__ void half2_project(const cx<R> (&pa)[3], const cx<R> (&pb)[3], int j, cx<R> (&h)[3]) {
+    // (MU 0) p1 - PM i p4 | p2 - PM i p3   (MU 1) p1 - PM p4 | p2 + PM p3
+    // (MU 2) p1 - PM i p3 | p2 + PM i p4   (MU 3) p1 - PM p3 | p2 - PM p4
+    const bool plus = (MU == 1 || MU == 2) && j == 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        cx<R> x = (MU == 0 || MU == 2) ? pmi<PM>(pb[c]) : pmr<PM>(pb[c]);
+        x.re = plus ? x.re : -x.re;
+        x.im = plus ? x.im : -x.im;
+        h[c] = cadd(pa[c], x);  // pa - x computed as pa + (-x): identical in IEEE arithmetic
+    }
+}
+
+template <int MU, int PM, typename R>
+__device__ __forceinline__ void half2_reconstruct(cx<R> (&a0)[3], cx<R> (&a2)[3], const cx<R> (&o)[3], int j) {
+    // spin j += o; spin (MU < 2 ? 3 - j : 2 + j) += y with
+    // (MU 0) +PM i o  (MU 1) j ? +PM o : -PM o  (MU 2) j ? -PM i o : +PM i o  (MU 3) -PM o
+    const bool minus = MU == 3 || (MU == 1 && j == 0) || (MU == 2 && j == 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a0[c] = cadd(a0[c], o[c]);
+        cx<R> y = (MU == 0 || MU == 2) ? pmi<PM>(o[c]) : pmr<PM>(o[c]);
+        y.re = minus ? -y.re : y.re;
+        y.im = minus ? -y.im : y.im;
+        a2[c] = cadd(a2[c], y);  // a - y computed as a + (-y)
+    }
+}
+
+__device__ __forceinline__ void pin6(cx<double> (&a)[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(a[k].re), "+v"(a[k].im)::"memory");
+}
+
+// 3 colours of spin `spin` of field q (one RHS) at site s
+__device__ __forceinline__ void ld_spin(const double2* q, int V, int spin, int s, cx<double> (&z)[3]) {
+    const double2* b = q + (size_t)(spin * 3) * V;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double2 v = b[(size_t)c * V + s];
+        z[c] = mk<double>(v.x, v.y);
+    }
+}
+
+template <int MU, int PM, bool FWD>
+__device__ __forceinline__ void hop2(cx<double> (&a0)[3], cx<double> (&a2)[3], const double2* q, const LinkF64& U,
+                                     const Geom& g, int s, const int x[4], int j) {
+    bool w;
+    const int t = FWD ? nbr_fwd(s, x, MU, g, w) : nbr_bwd(s, x, MU, g, w);
+    const bool ng = w && g.phase_neg[MU];
+    cx<double> u[9], pa[3], pb[3], h[3], o[3];
+    U.template ld9<double>(MU, FWD ? s : t, u);
+    ld_spin(q, g.V, j, t, pa);
+    ld_spin(q, g.V, MU < 2 ? 3 - j : 2 + j, t, pb);
+    half2_project<MU, PM>(pa, pb, j, h);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        h[c].re = ng ? -h[c].re : h[c].re;
+        h[c].im = ng ? -h[c].im : h[c].im;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (FWD)
+            o[a] = muladdmulti(u[3 * a], h[0], u[3 * a + 1], h[1], u[3 * a + 2], h[2]);
+        else
+            o[a] = muladdmulti(cconj(u[a]), h[0], cconj(u[3 + a]), h[1], cconj(u[6 + a]), h[2]);
+    }
+    half2_reconstruct<MU, PM>(a0, a2, o, j);
+    pin6(a0);
+    pin6(a2);
+}
+
+__device__ __forceinline__ double shfl_pair(double v) { return __shfl_xor(v, 1, 64); }
+
+template <int FPM, int MODE, int EPI>
+__global__ void __launch_bounds__(512) k_stream2(SpF64Out out, SpF64In in, LinkF64 U, SpF64In aux, Geom g,
+                                                 double kappa, int nrhs, int TL, int ZS, EpiArgs epi) {
+    static_assert(MODE == MODE_WILSON || MODE == MODE_DONLY, "k_stream2: Wilson and hopping-only operators");
+    if (epi.st != nullptr && epi.st->done) return;
+    if (epi.gate != nullptr && *epi.gate == 0) return;
+    __shared__ double spart[EPI == EPI_PAP ? kMaxTL : 1][EPI == EPI_PAP ? 256 : 1];
+    const int V3 = g.stride[3];
+    const int nblk3 = V3 >> 8;
+    const int ntc = g.L[3] / TL;
+    const int per = nblk3 / ZS;
+    int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int r = lb % nrhs;
+    lb /= nrhs;
+    const int b3lo = lb % per;
+    lb /= per;
+    const int tc = lb % ntc;
+    const int b3 = (lb / ntc) * per + b3lo;
+    const int i = threadIdx.x >> 1, j = threadIdx.x & 1;
+    const int s3 = (b3 << 8) + i;
+    const int V = g.V;
+    const double2* q = in.p + (size_t)r * 12 * V;
+    double2* qo = out.p + (size_t)r * 12 * V;
+    int x[4];
+    x[0] = s3 % g.L[0];
+    x[1] = (s3 / g.L[0]) % g.L[1];
+    x[2] = s3 / g.stride[2];
+    const int t0 = tc * TL;
+    // carried backward t-hop of the current slice, component j
+    cx<double> B[3];
+    {
+        x[3] = t0;
+        const int s = s3 + V3 * t0;
+        bool w;
+        const int tm = nbr_bwd(s, x, 3, g, w);
+        const bool ng = w && g.phase_neg[3];
+        cx<double> u[9], pa[3], pb[3], h[3];
+        U.template ld9<double>(3, tm, u);
+        ld_spin(q, V, j, tm, pa);
+        ld_spin(q, V, 2 + j, tm, pb);
+        half2_project<3, -FPM>(pa, pb, j, h);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            h[c].re = ng ? -h[c].re : h[c].re;
+            h[c].im = ng ? -h[c].im : h[c].im;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            B[a] = muladdmulti(cconj(u[a]), h[0], cconj(u[3 + a]), h[1], cconj(u[6 + a]), h[2]);
+    }
+    cx<double> ca[3], cb[3];  // this slice's spins j and 2 + j (carried from the t-forward hop)
+    for (int stp = 0; stp < TL; ++stp) {
+        const int t = t0 + stp;
+        x[3] = t;
+        const int s = s3 + V3 * t;
+        if (stp == 0) {
+            ld_spin(q, V, j, s, ca);
+            ld_spin(q, V, 2 + j, s, cb);
+        }
+        cx<double> a0[3], a2[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a0[c] = a2[c] = mk<double>(0.0, 0.0);
+        hop2<0, FPM, true>(a0, a2, q, U, g, s, x, j);
+        hop2<0, -FPM, false>(a0, a2, q, U, g, s, x, j);
+        hop2<1, FPM, true>(a0, a2, q, U, g, s, x, j);
+        hop2<1, -FPM, false>(a0, a2, q, U, g, s, x, j);
+        // spin 3 - j partial sums change hands: from here on lane j holds spin 2 + j
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a2[c].re = shfl_pair(a2[c].re);
+            a2[c].im = shfl_pair(a2[c].im);
+        }
+        hop2<2, FPM, true>(a0, a2, q, U, g, s, x, j);
+        hop2<2, -FPM, false>(a0, a2, q, U, g, s, x, j);
+        cx<double> ut[9], na[3], nb[3];
+        {
+            bool w;
+            const int tp = nbr_fwd(s, x, 3, g, w);
+            const bool ng = w && g.phase_neg[3];
+            cx<double> h[3], o[3];
+            U.template ld9<double>(3, s, ut);
+            ld_spin(q, V, j, tp, na);
+            ld_spin(q, V, 2 + j, tp, nb);
+            half2_project<3, FPM>(na, nb, j, h);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                h[c].re = ng ? -h[c].re : h[c].re;
+                h[c].im = ng ? -h[c].im : h[c].im;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) o[a] = muladdmulti(ut[3 * a], h[0], ut[3 * a + 1], h[1], ut[3 * a + 2], h[2]);
+            half2_reconstruct<3, FPM>(a0, a2, o, j);
+            half2_reconstruct<3, -FPM>(a0, a2, B, j);
+        }
+        // output spins j and 2 + j
+        cx<double> r0[3], r2[3];
+        if (MODE == MODE_DONLY) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                r0[c] = cadd(mk<double>(0.0, 0.0), rmul(0.5, a0[c]));
+                r2[c] = cadd(mk<double>(0.0, 0.0), rmul(0.5, a2[c]));
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                r0[c] = cadd(ca[c], rmul(-kappa, a0[c]));
+                r2[c] = cadd(cb[c], rmul(-kappa, a2[c]));
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            st_out(qo + (size_t)(j * 3 + c) * V + s, r0[c].re, r0[c].im);
+            st_out(qo + (size_t)((2 + j) * 3 + c) * V + s, r2[c].re, r2[c].im);
+        }
+        if constexpr (EPI == EPI_PAP) {
+            // Re sum_k conj(p_k) Ap_k in component order k = spin*3 + c: spins 0, 1, 2, 3
+            // alternate between the lanes of the pair, the running sum passed along
+            cx<double> pa[3], pb[3];
+            const double2* qp = aux.p + (size_t)r * 12 * V;
+            ld_spin(qp, V, j, s, pa);
+            ld_spin(qp, V, 2 + j, s, pb);
+            double red = 0.0;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const cx<double>* pv = half ? pb : pa;
+                const cx<double>* rv = half ? r2 : r0;
+                // lane 0 adds spin 2*half, then lane 1 adds spin 2*half + 1
+#pragma unroll
+                for (int turn = 0; turn < 2; ++turn) {
+                    if (j == turn) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            cx<double> tt = cmul(cconj(pv[c]), rv[c]);
+                            red = red + tt.re;
+                        }
+                    }
+                    red = shfl_pair(red);
+                }
+            }
+            // after the last pass lane 0 holds the site's sum
+            if (j == 0) spart[stp][i] = red;
+        }
+        // next step: this slice's backward t-hop from its own spins, and the carry
+        {
+            cx<double> h[3];
+            half2_project<3, -FPM>(ca, cb, j, h);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                B[a] = muladdmulti(cconj(ut[a]), h[0], cconj(ut[3 + a]), h[1], cconj(ut[6 + a]), h[2]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ca[c] = na[c];
+                cb[c] = nb[c];
+            }
+        }
+    }
+    if constexpr (EPI == EPI_PAP) {
+        // the per-step block trees of block_tree(): wave halving over each 64-site
+        // quarter, then (q0 + q2) + (q1 + q3); wave w takes steps w, w + 8, ...
+        __syncthreads();
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int stp = wave; stp < TL; stp += 8) {
+            double qs[4];
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) qs[qq] = wave_halve(spart[stp][qq * 64 + lane]);
+            if (lane == 0)
+                epi.partials[(long)r * epi.pstride + b3 + nblk3 * (t0 + stp)] = (qs[0] + qs[2]) + (qs[1] + qs[3]);
         }
     }
 }
@@ -513,6 +783,8 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
     const int nb_stream = (int)(ctx->V / 256 / sh.TL) * ngroups;
     const bool general = ctx->g.general != 0;
     hipStream_t st = ctx->stream;
+    // FP64 streaming Wilson / hopping-only launches use the two-lanes-per-site kernel
+    constexpr bool kTwoLane = LMX_TWO_LANE && std::is_same<In, SpF64In>::value && NR == 1;
     if (EPI != EPI_NONE && (donly || dag != (EPI != EPI_NORM))) {
         set_error("fused reduction %d not built for this operator (p.Ap and x/r updates on D^dagger, |D p|^2 on D)",
                   EPI);
@@ -524,6 +796,13 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
     }
 #define LMX_GO(FPM, MODE)                                                                                      \
     do {                                                                                                       \
+        if constexpr (kTwoLane && MODE != MODE_CLOVER && (EPI == EPI_NONE || EPI == EPI_PAP)) {                \
+            if (!general && sh.stream) {                                                                       \
+                hipLaunchKernelGGL((k_stream2<FPM, MODE, EPI>), dim3(nb_stream), dim3(512), 0, st, out, in, U, aux, \
+                                   ctx->g, kappa, nrhs, sh.TL, sh.ZS, epi);                                    \
+                break;                                                                                         \
+            }                                                                                                  \
+        }                                                                                                      \
         if (general || (!sh.stream && !SITE_NG))                                                               \
             hipLaunchKernelGGL((k_site<In, Lk, Out, Cl, Aux, Ex, FPM, MODE, NR, EPI, true>), dim3(nb_site),    \
                                dim3(256), 0, st, out, in, U, F, aux, ex, ctx->g, kappa, ccoef, ngroups, nrhs, epi); \
