@@ -182,6 +182,17 @@ int lmx_clover_export(lmx_wilson* op, void* F_out);
  * (src/Operators/WilsonDiracOperator.jl:60-76,425-441,535-551,646-662) and the
  * clover mul! (src/Operators/WilsonDiracCloverOperator.jl:561-601). */
 int lmx_wilson_apply(lmx_wilson* op, void* out, const void* in, int flags, int nrhs);
+/* The same mul! directly on the reference's storage, without conversion:
+ * psi and C are LatticeMatrix.A arrays (complex128, column-major
+ * [3,4,L1+2nw,L2+2nw,L3+2nw,L4+2nw], nrhs of them stacked).  psi's halo is
+ * not read (neighbours wrap in-kernel with the context's spinor phases,
+ * which is what set_halo! would have put there); only C's core sites are
+ * written, its halo is left as it was (stale, as after the reference's
+ * mul!).  Replaces mul!(C::LatticeMatrix, D, psi) / mul!(C, D', psi) with
+ * the LatticeMatrix arrays passed in place
+ * (src/Operators/WilsonDiracOperator.jl:60-76,425-441; clover
+ * src/Operators/WilsonDiracCloverOperator.jl:561-601). */
+int lmx_wilson_apply_padded(lmx_wilson* op, void* C, const void* psi, int nw, int flags, int nrhs);
 /* out = D^dagger D in with caller temp -- DdagDOp(D, temp) mul!,
  * src/Operators/DiracOperators.jl:104-121 (temp must not alias in/out). */
 int lmx_ddagd_apply(lmx_wilson* op, void* out, const void* in, void* tmp, int nrhs);

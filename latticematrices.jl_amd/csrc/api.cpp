@@ -369,6 +369,16 @@ int lmx_wilson_apply(lmx_wilson* op, void* out, const void* in, int flags, int n
                              (double2*)out, (const double2*)in, nrhs, flags, e);
 }
 
+int lmx_wilson_apply_padded(lmx_wilson* op, void* C, const void* psi, int nw, int flags, int nrhs) {
+    clear_error();
+    LMX_TRY(check_op(op));
+    LMX_CHECK(C && psi && nrhs >= 0 && nw >= 0, LMX_ERR_ARG, "bad wilson_apply_padded arguments");
+    LMX_CHECK(C != psi, LMX_ERR_ALIAS, "output and input of the Wilson operator must not alias");
+    LMX_CHECK((flags & ~(LMX_DAGGER | LMX_DONLY)) == 0, LMX_ERR_ARG, "unknown flags %d", flags);
+    return launch_wilson_padded(op->ctx, op->links, (flags & LMX_DONLY) ? nullptr : op->clover64, op->kappa, op->csw,
+                                (double2*)C, (const double2*)psi, nw, nrhs, flags);
+}
+
 int lmx_ddagd_apply(lmx_wilson* op, void* out, const void* in, void* tmp, int nrhs) {
     clear_error();
     LMX_TRY(check_op(op));

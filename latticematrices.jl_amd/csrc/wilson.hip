@@ -70,3 +70,142 @@ int launch_wilson_lo(lmx_context* ctx, const LoArgs& a) {
 }
 
 }  // namespace lmx
+
+namespace lmx {
+
+// ---------------------------------------------------------------------------
+// mul! on the reference's own storage: psi and C are LatticeMatrix.A arrays
+// (complex128, column-major [3,4,L1+2nw,...,L4+2nw], colour fastest, i.e.
+// component spin*3 + colour of padded site p at p*12 + k).  psi's halo is not
+// read: neighbours wrap in-kernel with the spinor phases (the halo fill of
+// set_halo!, src/LatticeMatrices_core.jl:707-723, applied on the fly); only
+// C's core sites are written.  Links and the clover cache are the operator's
+// own (core-indexed).  Arithmetic as k_site with general phases.
+// ---------------------------------------------------------------------------
+struct PadGeom {
+    int PS[4];  // padded strides (1, P1, P1 P2, P1 P2 P3)
+    int nw;
+    long Vp;    // padded volume
+};
+
+template <int MU, int PM, bool FWD, typename R>
+__device__ __forceinline__ void hop_pad(cx<R> (&acc)[12], const double2* __restrict__ q, const LinkF64& U,
+                                        const Geom& g, const PadGeom& pg, int s, long sp, const int x[4]) {
+    bool w;
+    const int t = FWD ? nbr_fwd(s, x, MU, g, w) : nbr_bwd(s, x, MU, g, w);
+    const long step = (long)pg.PS[MU];
+    const long tp = FWD ? (w ? sp - (long)(g.L[MU] - 1) * step : sp + step)
+                        : (w ? sp + (long)(g.L[MU] - 1) * step : sp - step);
+    cx<R> u[9], psi[12], h[6], o[6];
+    U.template ld9<R>(MU, FWD ? s : t, u);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        double2 v = q[tp * 12 + k];
+        psi[k] = mk<R>(v.x, v.y);
+    }
+    if (w && !g.phase_one[MU]) {
+        if (FWD)
+            phase_spinor(psi, g.ph_re[MU], g.ph_im[MU]);
+        else
+            phase_spinor(psi, g.iph_re[MU], g.iph_im[MU]);
+    }
+    half_project<PM, MU>(psi, h);
+    if (FWD)
+        matvec_fwd(u, h, o);
+    else
+        matvec_bwd(u, h, o);
+    reconstruct_add<PM, MU>(acc, o);
+    pin_acc(acc);
+}
+
+template <int FPM, int MODE>
+__global__ void __launch_bounds__(256) k_wilson_pad(double2* __restrict__ C, const double2* __restrict__ psi,
+                                                    LinkF64 U, F64Clover F, Geom g, PadGeom pg, double kappa,
+                                                    double ccoef, int nrhs) {
+    const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= (long)g.V * nrhs) return;
+    const int s = (int)(gid % g.V);
+    const int r = (int)(gid / g.V);
+    int x[4];
+    site_coords(s, g, x);
+    const long sp = (long)(x[0] + pg.nw) + (long)pg.PS[1] * (x[1] + pg.nw) + (long)pg.PS[2] * (x[2] + pg.nw) +
+                    (long)pg.PS[3] * (x[3] + pg.nw);
+    const double2* q = psi + (long)r * pg.Vp * 12;
+    cx<double> acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = mk<double>(0.0, 0.0);
+    hop_pad<0, FPM, true>(acc, q, U, g, pg, s, sp, x);
+    hop_pad<0, -FPM, false>(acc, q, U, g, pg, s, sp, x);
+    hop_pad<1, FPM, true>(acc, q, U, g, pg, s, sp, x);
+    hop_pad<1, -FPM, false>(acc, q, U, g, pg, s, sp, x);
+    hop_pad<2, FPM, true>(acc, q, U, g, pg, s, sp, x);
+    hop_pad<2, -FPM, false>(acc, q, U, g, pg, s, sp, x);
+    hop_pad<3, FPM, true>(acc, q, U, g, pg, s, sp, x);
+    hop_pad<3, -FPM, false>(acc, q, U, g, pg, s, sp, x);
+    cx<double> res[12];
+    if (MODE == MODE_DONLY) {
+#pragma unroll
+        for (int sp4 = 0; sp4 < 4; ++sp4)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) res[sp4 * 3 + c] = cadd(mk<double>(0.0, 0.0), rmul(0.5, acc[c * 4 + sp4]));
+    } else {
+        cx<double> p[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            double2 v = q[sp * 12 + k];
+            p[k] = mk<double>(v.x, v.y);
+        }
+#pragma unroll
+        for (int sp4 = 0; sp4 < 4; ++sp4)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) res[sp4 * 3 + c] = cadd(p[sp4 * 3 + c], rmul(-kappa, acc[c * 4 + sp4]));
+        if constexpr (MODE == MODE_CLOVER) clover_add<double>(F, s, g.V, p, ccoef, res);
+    }
+    double2* o = C + (long)r * pg.Vp * 12 + sp * 12;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) o[k] = make_double2(res[k].re, res[k].im);
+}
+
+int launch_wilson_padded(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
+                         double2* C, const double2* psi, int nw, int nrhs, int flags) {
+    if (ctx->V == 0 || nrhs == 0) return LMX_OK;
+    const Geom& g = ctx->g;
+    PadGeom pg;
+    pg.nw = nw;
+    long st = 1;
+    for (int d = 0; d < 4; ++d) {
+        pg.PS[d] = (int)st;
+        st *= g.L[d] + 2 * nw;
+    }
+    pg.Vp = st;
+    LMX_CHECK(st * 12 < (1L << 40), LMX_ERR_ARG, "padded volume too large");
+    const double ccoef = (-kappa) * csw;
+    const long n = ctx->V * (long)nrhs;
+    dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+    LinkF64 l{U, (int)ctx->V};
+    F64Clover F{clover};
+    const bool dag = (flags & LMX_DAGGER) != 0, donly = (flags & LMX_DONLY) != 0;
+#define LMX_PAD(FPM, MODE) \
+    hipLaunchKernelGGL((k_wilson_pad<FPM, MODE>), grid, blk, 0, ctx->stream, C, psi, l, F, g, pg, kappa, ccoef, nrhs)
+    if (donly) {
+        if (dag)
+            LMX_PAD(1, MODE_DONLY);
+        else
+            LMX_PAD(-1, MODE_DONLY);
+    } else if (clover) {
+        if (dag)
+            LMX_PAD(1, MODE_CLOVER);
+        else
+            LMX_PAD(-1, MODE_CLOVER);
+    } else {
+        if (dag)
+            LMX_PAD(1, MODE_WILSON);
+        else
+            LMX_PAD(-1, MODE_WILSON);
+    }
+#undef LMX_PAD
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+}  // namespace lmx

@@ -8,12 +8,12 @@ from . import _lib, generators
 from ._lib import ArgumentError, LmxError
 from .lattice import (Adjoint, CGResult, DdagDOp, GaugeField, Lattice, ScaledOperator, SpinorField,
                       WilsonDiracCloverOperator4D, WilsonDiracOperator4D, WilsonDiracOperator4D_Donly, adjoint,
-                      axpby_, cg_, clear_matrix_, dot, dSFdU, mul_, pseudofermion_action, solve_, substitute_,
+                      axpby_, cg_, clear_matrix_, dot, dSFdU, mul_, mul_padded_, pseudofermion_action, solve_, substitute_,
                       ta_project_, wilson_pullback_, cg_batched_, autotune)
 
 __all__ = [
     "ArgumentError", "LmxError", "Adjoint", "CGResult", "DdagDOp", "GaugeField", "Lattice", "ScaledOperator",
     "SpinorField", "WilsonDiracCloverOperator4D", "WilsonDiracOperator4D", "WilsonDiracOperator4D_Donly",
-    "adjoint", "axpby_", "cg_", "clear_matrix_", "dot", "dSFdU", "mul_", "pseudofermion_action", "solve_",
+    "adjoint", "axpby_", "cg_", "clear_matrix_", "dot", "dSFdU", "mul_", "mul_padded_", "pseudofermion_action", "solve_",
     "substitute_", "ta_project_", "wilson_pullback_", "generators", "cg_batched_", "autotune",
 ]

@@ -294,3 +294,46 @@ def test_full_volume_bit_exact(gpu):
         lmx.mul_(y, lmx.adjoint(D) if dag else D, x)
         ref = orc.wilson(U, psi, L, 0.124, dagger=dag)
         assert np.array_equal(y.to_numpy(), ref), f"dagger={dag} max rel {rel(y.to_numpy(), ref):.3e}"
+
+
+@pytest.mark.parametrize("nw,phases,csw", [(1, (1, 1, 1, -1), 0.0), (2, (np.exp(0.4j), -1, 1j, -1), 0.0),
+                                           (1, (1, 1, 1, -1), 1.2)])
+def test_apply_on_reference_storage(gpu, nw, phases, csw):
+    # mul! directly on LatticeMatrix.A-layout arrays: garbage in psi's halo is
+    # never read, C's halo is left untouched, the core is bit-exact
+    import torch
+    L = (4, 6, 2, 8)
+    print(f"padded apply nw={nw} phases={phases} csw={csw}", flush=True)
+    U = gen.links(L, "haar", 81)
+    rng = np.random.default_rng(82)
+    P = [d + 2 * nw for d in L]
+    nrhs = 2
+    shape = (nrhs, P[3], P[2], P[1], P[0], 4, 3)
+    pad = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)  # halo: garbage
+    core = (slice(None),) + tuple(slice(nw, nw + d) for d in reversed(L)) + (slice(None), slice(None))
+    psi_core = np.ascontiguousarray(pad[core]).reshape(nrhs, -1, 4, 3)
+    lat = lmx.Lattice(L, phases)
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracOperator4D(Ug, 0.118, csw)
+    F = orc.clover_field_strength(U, L) if csw else None
+    psi_d = torch.from_numpy(pad.copy()).to(lat.device)
+    sentinel = np.full(shape, 7.0 - 3.0j)
+    for flags in ((0, 1, 2, 3) if csw == 0 else (0, 1)):
+        A = lmx.WilsonDiracOperator4D_Donly(Ug) if flags & 2 else D
+        if flags & 1:
+            A = lmx.adjoint(A)
+        C_d = torch.from_numpy(sentinel.copy()).to(lat.device)
+        lmx.mul_padded_(C_d, A, psi_d, nw)
+        C = C_d.cpu().numpy()
+        for i in range(nrhs):
+            if csw:
+                ref = orc.wilson_clover(U, F, psi_core[i], L, 0.118, csw, phases=phases, dagger=bool(flags & 1))
+            else:
+                ref = orc.wilson(U, psi_core[i], L, 0.118, phases, dagger=bool(flags & 1), donly=bool(flags & 2))
+            got = np.ascontiguousarray(C[i][core[1:]]).reshape(-1, 4, 3)
+            assert np.array_equal(got, ref), f"flags={flags} rhs={i} max rel {rel(got, ref):.3e}"
+        halo = np.ones(shape, dtype=bool)
+        halo[core] = False
+        assert np.all(C[halo] == sentinel[halo])
+    with pytest.raises(lmx.ArgumentError):
+        lmx.mul_padded_(psi_d, D, psi_d, nw)
