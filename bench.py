@@ -93,11 +93,15 @@ def measured_traffic(L):
 
 
 def lo_dslash_bytes(precision, n):
-    """Algorithmic bytes per site per RHS of the low-precision dslash: the
-    links once per launch of n RHS, the input and output spinor once each."""
+    """Algorithmic bytes per site per RHS of one operator application of the
+    mixed solver's fused pair (the autotuner times D with |D p|^2 and
+    D^dagger with the x / r update, and reports the pair / 2): links once per
+    launch of n RHS in each; D reads p, writes t; D^dagger reads t, p, r,
+    x_inner (FP32) and writes r, x_inner."""
     links = 288.0 if precision == "f32" else 144.0
     spinor = 52.0 if precision == "h16" else 96.0
-    return links / n + 2 * spinor
+    pair = 2 * links / n + 2 * spinor + 4 * spinor + 2 * 96.0
+    return pair / 2
 
 
 def copy_calibration(lat):
@@ -165,8 +169,8 @@ def main():
     if args.mode == "kernels":
         tune = lmx.autotune(D, args.precision, n, None)
         bytes_dm = lo_dslash_bytes(args.precision, n)
-        log(f"bench: {args.precision} dslash x{n}: {tune['ms_per_apply'] * 1e3:.1f} us/RHS, "
-            f"{bytes_dm * V / (tune['ms_per_apply'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
+        log(f"bench: {args.precision} fused D / D^dagger pair x{n}: {tune['ms_per_apply'] * 1e3:.1f} us per application "
+            f"per RHS, {bytes_dm * V / (tune['ms_per_apply'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
         print(json.dumps({"mode": "kernels", "fp64_dslash_us": t_d64 * 1e3, "fp64_dslash_gbs": ach_d64,
                           "copy_1gib_us": t_copy * 1e3, "lo_dslash_us_per_rhs": tune["ms_per_apply"] * 1e3,
                           "lo_rhs_block": tune["rhs_block"]}), flush=True)
@@ -191,8 +195,10 @@ def main():
     tune = lmx.autotune(D, args.precision, n, os.path.join(ROOT, "gpurun_out", "lmx_tune.txt"))
     bytes_dm = lo_dslash_bytes(args.precision, n)
     ach_dm = bytes_dm * V / (tune["ms_per_apply"] * 1e-3) / 1e9
-    log(f"bench: autotuned {args.precision} dslash x{n}: rhs_block {tune['rhs_block'] >> 4}/{tune['rhs_block'] & 15} "
-        f"{tune['ms_per_apply'] * 1e3:.1f} us/RHS, {ach_dm:.0f} GB/s algorithmic")
+    blk = tune["rhs_block"]
+    log(f"bench: autotuned {args.precision} fused D / D^dagger pair x{n}: TL {blk >> 12}, {(blk >> 4) & 0xff} RHS per "
+        f"launch, {blk & 15} per work item: {tune['ms_per_apply'] * 1e3:.1f} us per application per RHS, "
+        f"{ach_dm:.0f} GB/s algorithmic")
     xm = lat.spinor(n)
 
     def solve_mixed():
@@ -251,7 +257,8 @@ def main():
         "mixed_rhs0_error_vs_fp64": err0,
         "dslash_gbs": ach_d64,
         "copy_1gib_us": t_copy * 1e3,
-        "mixed_dslash": {"rhs_block": tune["rhs_block"], "us_per_rhs": tune["ms_per_apply"] * 1e3,
+        "mixed_dslash": {"kernels": "D with |Dp|^2 + D^dagger with the fused x/r update, per application",
+                         "rhs_block": tune["rhs_block"], "us_per_rhs": tune["ms_per_apply"] * 1e3,
                          "algorithmic_bytes_per_site_per_rhs": bytes_dm, "achieved_gbs": ach_dm,
                          "frac": ach_dm / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "achieved": ach_d64, "peak": HBM_PEAK_GBS, "unit": "GB/s",

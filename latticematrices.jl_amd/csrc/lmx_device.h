@@ -154,6 +154,15 @@ __device__ __forceinline__ void st_out(int4* p, int4 w) {
     *p = w;
 #endif
 }
+__device__ __forceinline__ void st_out(float4* p, float4 w) {
+#if LMX_NT_STORE
+    typedef float lmx_fv4 __attribute__((ext_vector_type(4)));
+    lmx_fv4 v = {w.x, w.y, w.z, w.w};
+    __builtin_nontemporal_store(v, (lmx_fv4*)p);
+#else
+    *p = w;
+#endif
+}
 __device__ __forceinline__ void st_out(float* p, float v) {
 #if LMX_NT_STORE
     __builtin_nontemporal_store(v, p);
@@ -233,6 +242,40 @@ struct SpF32Out {
         float2* q = p + (size_t)r * 12 * V;
 #pragma unroll
         for (int k = 0; k < 12; ++k) st_out((float2*)((char*)(q + (size_t)k * V) + (unsigned)s * 8u), z[k].re, z[k].im);
+    }
+};
+
+// FP32 spinor in component pairs: float4 [nrhs][6][V] holding components
+// (2m, 2m+1) -- the mixed solver's inner solution (half the instructions of
+// the float2 layout for the same bytes).
+struct SpX4In {
+    using R = float;
+    const float4* __restrict__ p;
+    int V;
+    static SpX4In at(const void* q, const float*, int V, int r0) {
+        return SpX4In{(const float4*)q + (size_t)r0 * 6 * V, V};
+    }
+    __device__ __forceinline__ void ld12(int r, int s, cx<float>* z) const {
+        const float4* q = p + (size_t)r * 6 * V;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            float4 v = ldv(q + (size_t)m * V, s);
+            z[2 * m] = mk<float>(v.x, v.y);
+            z[2 * m + 1] = mk<float>(v.z, v.w);
+        }
+    }
+};
+struct SpX4Out {
+    using R = float;
+    float4* __restrict__ p;
+    int V;
+    static SpX4Out at(void* q, float*, int V, int r0) { return SpX4Out{(float4*)q + (size_t)r0 * 6 * V, V}; }
+    __device__ __forceinline__ void st12(int r, int s, const cx<float>* z) const {
+        float4* q = p + (size_t)r * 6 * V;
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+            st_out((float4*)((char*)(q + (size_t)m * V) + (unsigned)s * 16u),
+                   make_float4(z[2 * m].re, z[2 * m].im, z[2 * m + 1].re, z[2 * m + 1].im));
     }
 };
 

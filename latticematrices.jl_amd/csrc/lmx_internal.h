@@ -133,7 +133,7 @@ struct LoArgs {
     int epi;
     void* r;
     float* r_norm;
-    float2* x;
+    void* x;  // FP32 component pairs, float4 [nrhs][6][V]
     const double* alpha;
     int astride;
     const int* done;

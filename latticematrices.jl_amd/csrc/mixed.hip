@@ -47,7 +47,7 @@ struct InnerVec {
 };
 
 struct MixPtrs {
-    float2* xlo;
+    float4* xlo;  // inner solution: FP32 component pairs, float4 [nrhs][6][V]
     InnerVec r, p, t;
     double2* r64;
     double2* t64;
@@ -80,7 +80,7 @@ static size_t layout(lmx_context* ctx, int nrhs, int sp, MixPtrs* P, char* base)
         return v;
     };
     MixPtrs q;
-    q.xlo = (float2*)take(f32);
+    q.xlo = (float4*)take(f32);
     q.r = inner();
     q.p = inner();
     q.t = inner();
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ parti
 // format); x_lo = 0; |r|^2 and |b|^2 per RHS.
 template <class SO>
 __global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, double2* __restrict__ r64, SO rlo,
-                                                 SO p, float2* __restrict__ xlo, int V, double* partials) {
+                                                 SO p, float4* __restrict__ xlo, int V, double* partials) {
     const int i = blockIdx.y;
     const int s = blockIdx.x * 256 + threadIdx.x;
     double rr = 0.0, bb = 0.0;
@@ -200,12 +200,13 @@ __global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, 
             double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
             r64[e] = rv;
             f[k] = mk<float>((float)rv.x, (float)rv.y);
-            xlo[e] = make_float2(0.f, 0.f);
             rr += rv.x * rv.x + rv.y * rv.y;
             bb += bv.x * bv.x + bv.y * bv.y;
         }
         rlo.st12(i, s, f);
         p.st12(i, s, f);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) xlo[(size_t)i * 6 * V + (size_t)m * V + s] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     row_partials(rr, bb, partials);
 }
@@ -227,20 +228,21 @@ __global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta,
 }
 
 // reliable update, part 1: x += x_lo (FP64), x_lo = 0
-__global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float2* __restrict__ xlo, int V,
+__global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float4* __restrict__ xlo, int V,
                                                   const MixRhs* rhs, const MixBatch* batch, int force) {
     const int i = blockIdx.y;
     if ((!force && !batch->do_update) || rhs[i].done) return;
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= V) return;
-    const size_t o = (size_t)i * 12 * V;
+    const size_t o = (size_t)i * 12 * V, o4 = (size_t)i * 6 * V;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-        const size_t e = o + (size_t)k * V + s;
-        float2 l = xlo[e];
-        double2 h = x[e];
-        x[e] = make_double2(h.x + (double)l.x, h.y + (double)l.y);
-        xlo[e] = make_float2(0.f, 0.f);
+    for (int m = 0; m < 6; ++m) {
+        const float4 l = xlo[o4 + (size_t)m * V + s];
+        const size_t e0 = o + (size_t)(2 * m) * V + s, e1 = e0 + V;
+        const double2 h0 = x[e0], h1 = x[e1];
+        x[e0] = make_double2(h0.x + (double)l.x, h0.y + (double)l.y);
+        x[e1] = make_double2(h1.x + (double)l.z, h1.y + (double)l.w);
+        xlo[o4 + (size_t)m * V + s] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
@@ -559,9 +561,10 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
 }
 
 // ---------------------------------------------------------------------------
-// Autotuner: time the inner (low-precision) dslash for every RHS blocking
-// (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,4}) and
-// streaming depth (t-slices per workgroup: automatic, 4, 8) and keep
+// Autotuner: time one inner iteration's operator pair as the solver runs it
+// (D with |D p|^2, D^dagger with the fused x / r update) for every RHS
+// blocking (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,4})
+// and streaming depth (t-slices per workgroup: automatic, 4, 8) and keep
 // the fastest per (volume, precision, nrhs, arch); optionally persisted as
 // text lines "V precision nrhs arch rhs_block us_per_rhs".
 // ---------------------------------------------------------------------------
@@ -615,26 +618,62 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     }
     LMX_TRY(ensure_lo_links(op));
     const int sp = spinor_prec_of(precision);
-    const size_t fb = lmx_spinor_bytes(c, sp, nrhs);
-    char *in = nullptr, *o = nullptr;
-    LMX_HIP(hipMalloc(&in, fb));
-    if (hipMalloc(&o, fb) != hipSuccess) {
-        (void)hipFree(in);
-        set_error("autotune buffer allocation failed");
+    // buffers of one inner iteration: p, t = D p, r, x_inner, alpha, done, partials
+    const size_t fb = align256(lmx_spinor_bytes(c, sp, nrhs));
+    const size_t xb = align256(12 * (size_t)c->V * nrhs * sizeof(float2));
+    const size_t nbk = (size_t)nblocks(c->V);
+    const size_t pb = align256(2 * nbk * nrhs * sizeof(double));
+    const size_t ab = align256(nrhs * sizeof(double)), db = align256(nrhs * sizeof(int));
+    const size_t total = 3 * fb + xb + pb + ab + db;
+    char* buf = nullptr;
+    if (hipMalloc(&buf, total) != hipSuccess) {
+        set_error("autotune buffer allocation of %zu bytes failed", total);
         return LMX_ERR_NOMEM;
     }
-    (void)hipMemsetAsync(in, 0, fb, c->stream);
+    (void)hipMemsetAsync(buf, 0, total, c->stream);
+    char* pbuf = buf;
+    char* tbuf = buf + fb;
+    char* rbuf = buf + 2 * fb;
+    char* xbuf = buf + 3 * fb;
+    double* parts = (double*)(xbuf + xb);
+    const double* alpha = (const double*)((char*)parts + pb);
+    const int* done = (const int*)((const char*)alpha + ab);
+    auto norm_of = [&](char* q) -> float* {  // per-site scales follow the 16-bit data
+        return sp == LMX_H16 ? (float*)(q + (size_t)nrhs * c->V * 3 * sizeof(int4)) : nullptr;
+    };
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    LoArgs la = lo_args(op, link_prec_of(precision), sp);
-    la.nrhs = nrhs;
-    la.in = in;
-    la.out = o;
-    if (sp == LMX_H16) {  // per-site scales follow the 16-bit data
-        la.in_norm = (const float*)(in + (size_t)nrhs * c->V * 3 * sizeof(int4));
-        la.out_norm = (float*)(o + (size_t)nrhs * c->V * 3 * sizeof(int4));
-    }
+    LoArgs ld = lo_args(op, link_prec_of(precision), sp);  // t = D p, |t|^2
+    ld.nrhs = nrhs;
+    ld.in = pbuf;
+    ld.in_norm = norm_of(pbuf);
+    ld.out = tbuf;
+    ld.out_norm = norm_of(tbuf);
+    ld.epi = 2;
+    ld.partials = parts;
+    ld.pstride = 2L * (long)nbk;
+    LoArgs lx = lo_args(op, link_prec_of(precision), sp);  // D^dagger t with the x / r update
+    lx.nrhs = nrhs;
+    lx.in = tbuf;
+    lx.in_norm = norm_of(tbuf);
+    lx.aux = pbuf;
+    lx.aux_norm = norm_of(pbuf);
+    lx.flags = LMX_DAGGER;
+    lx.epi = 3;
+    lx.r = rbuf;
+    lx.r_norm = norm_of(rbuf);
+    lx.x = xbuf;
+    lx.alpha = alpha;
+    lx.astride = 1;
+    lx.done = done;
+    lx.dstride = 1;
+    lx.partials = parts;
+    lx.pstride = 2L * (long)nbk;
+    auto pair = [&]() {
+        int rc2 = launch_wilson_lo(c, ld);
+        return rc2 == LMX_OK ? launch_wilson_lo(c, lx) : rc2;
+    };
     const int per_opts[5] = {1, 2, 4, 6, 12};
     int rc = LMX_OK;
     best = 0;
@@ -651,16 +690,16 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
         for (int nr = 1; nr <= 4 && rc == LMX_OK; ++nr) {
             if (nr == 3 || nr > per || per % nr != 0) continue;  // built: 1, 2, 4 per work item
             const int blk = (tl << 12) | (per << 4) | nr;
-            la.rhs_block = blk;
-            for (int k = 0; k < 2 && rc == LMX_OK; ++k) rc = launch_wilson_lo(c, la);
+            ld.rhs_block = lx.rhs_block = blk;
+            for (int k = 0; k < 2 && rc == LMX_OK; ++k) rc = pair();
             const int reps = 5;
             (void)hipEventRecord(e0, c->stream);
-            for (int k = 0; k < reps && rc == LMX_OK; ++k) rc = launch_wilson_lo(c, la);
+            for (int k = 0; k < reps && rc == LMX_OK; ++k) rc = pair();
             (void)hipEventRecord(e1, c->stream);
             if (hipEventSynchronize(e1) != hipSuccess) rc = LMX_ERR_HIP;
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            const double us = 1e3 * ms / reps / nrhs;
+            const double us = 1e3 * ms / reps / nrhs / 2;  // per operator application and RHS
             if (rc == LMX_OK && us < best_us) {
                 best_us = us;
                 best = blk;
@@ -670,8 +709,7 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(in);
-    (void)hipFree(o);
+    (void)hipFree(buf);
     LMX_TRY(rc);
     op->tuned_block[precision][nrhs] = best;
     if (cache_path) {

@@ -178,8 +178,8 @@ template <class In, class Out>
 struct XrEx {
     In rin;        // r (read) ...
     Out rout;      // ... and written in place
-    SpF32In xin;   // inner solution x (FP32)
-    SpF32Out xout;
+    SpX4In xin;    // inner solution x (FP32, component pairs)
+    SpX4Out xout;
     const double* alpha;  // alpha of RHS r at alpha[r * astride]
     int astride;
     const int* done;      // retired RHS (no update) at done[r * dstride]
@@ -214,12 +214,15 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
         ex.rin.ld12(r, s, rv);
         aux.ld12(r, s, pv);
         ex.xin.ld12(r, s, xv);
+        R site = R(0);  // per-site sum in working precision, accumulated over sites in FP64
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
             rv[k] = mk<R>(fmaR(-a, res[k].re, rv[k].re), fmaR(-a, res[k].im, rv[k].im));
             xv[k] = mk<R>(fmaR(a, pv[k].re, xv[k].re), fmaR(a, pv[k].im, xv[k].im));
-            red += double(rv[k].re) * double(rv[k].re) + double(rv[k].im) * double(rv[k].im);
+            site = fmaR(rv[k].re, rv[k].re, site);
+            site = fmaR(rv[k].im, rv[k].im, site);
         }
+        red = double(site);
         if (store && ex.done[(long)r * ex.dstride] == 0) {
             ex.rout.st12(r, s, rv);
             ex.xout.st12(r, s, xv);
@@ -240,9 +243,13 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
         }
     }
     if constexpr (EPI == EPI_NORM) {
+        R site = R(0);  // per-site sum in working precision, accumulated over sites in FP64
 #pragma unroll
-        for (int k = 0; k < 12; ++k)
-            red += double(res[k].re) * double(res[k].re) + double(res[k].im) * double(res[k].im);
+        for (int k = 0; k < 12; ++k) {
+            site = fmaR(res[k].re, res[k].re, site);
+            site = fmaR(res[k].im, res[k].im, site);
+        }
+        red = double(site);
     }
     return red;
 }
@@ -870,7 +877,7 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
         if (a.epi == EPI_XR) {
             In p = In::at(a.aux, a.aux_norm, V, r0);
             XrEx<In, Out> ex{In::at(a.r, a.r_norm, V, r0), Out::at(a.r, a.r_norm, V, r0),
-                             SpF32In::at(a.x, nullptr, V, r0), SpF32Out::at(a.x, nullptr, V, r0),
+                             SpX4In::at(a.x, nullptr, V, r0), SpX4Out::at(a.x, nullptr, V, r0),
                              a.alpha + (long)r0 * a.astride, a.astride, a.done + (long)r0 * a.dstride, a.dstride};
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, XrEx<In, Out>, NR, EPI_XR, false, false>(
                 ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl)));
