@@ -236,7 +236,11 @@ int lmx_cg_batched(lmx_wilson* op, void* x, const void* b, int nrhs, void* works
  * dresult and psi (F64 spinors) -- the Enzyme reverse rule kernel
  * _kernel_wilson_link_pullback! (ext/AD/WilsonDiracOperator.jl:62-91,137-143)
  * with _wilson_halfspin_link_pullback_direction3!
- * (src/Operators/WilsonDiracOperator.jl:354-410).  dU: internal F64 links layout. */
+ * (src/Operators/WilsonDiracOperator.jl:354-410).  For a Wilson-clover
+ * operator (csw != 0) the clover part is added with coefficient * csw, as
+ * wilson_clover_link_pullback!(dU, D, U, dresult, psi; coefficient = scale)
+ * does with -kappa*scale and -kappa*cSW*scale
+ * (src/Operators/WilsonCloverPullback.jl:349-389).  dU: internal F64 links layout. */
 int lmx_wilson_pullback(lmx_wilson* op, void* dU, const void* dresult, const void* psi, double coefficient);
 
 /* F[mu](x) = TA(U[mu](x) G[mu](x)^dagger) (kernel_traceless_antihermitian!,
@@ -244,7 +248,8 @@ int lmx_wilson_pullback(lmx_wilson* op, void* dU, const void* dresult, const voi
 int lmx_ta_project(lmx_context* ctx, void* F, const void* links, const void* G);
 
 /* dSFdU(dfdU, D, phi) (ext/LatticeMatricesEnzymeExt.jl:395-460) for the
- * Wilson operator: eta = (D^dagger D)^-1 phi by FP64 CG from zero, chi = D eta,
+ * Wilson or Wilson-clover operator: eta = (D^dagger D)^-1 phi by FP64 CG
+ * from zero, chi = D eta,
  * G = pullback(dresult = -2 chi, eta, coefficient -kappa) (the Enzyme
  * convention G = dS/dReU + i dS/dImU), and, if F is non-NULL, the su(3) force
  * F = TA(U G^dagger).  eta/chi: caller F64 spinors (outputs).  work: 4 spinors. */

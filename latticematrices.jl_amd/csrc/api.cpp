@@ -331,6 +331,7 @@ int lmx_wilson_destroy(lmx_wilson* op) {
     if (op->clover64) (void)hipFree(op->clover64);
     if (op->clover32) (void)hipFree(op->clover32);
     if (op->clover16) (void)hipFree(op->clover16);
+    if (op->clover_dF) (void)hipFree(op->clover_dF);
     delete op;
     return LMX_OK;
 }
@@ -487,11 +488,22 @@ int lmx_cg_batched(lmx_wilson* op, void* x, const void* b, int nrhs, void* work,
     return cg_batched_impl(op, (double2*)x, (const double2*)b, nrhs, work, bytes, params, results);
 }
 
+namespace lmx {
+// Hopping pullback, plus the clover part (coefficient * csw) for a clover operator.
+static int operator_pullback(lmx_wilson* op, double2* dU, const double2* dres, const double2* psi,
+                             double coefficient) {
+    LMX_TRY(launch_pullback_f64(op->ctx, dU, dres, psi, coefficient));
+    if (op->csw == 0.0) return LMX_OK;
+    if (!op->clover_dF) LMX_HIP(hipMalloc(&op->clover_dF, 6 * 9 * (size_t)op->ctx->V * sizeof(double2)));
+    return launch_clover_pullback(op->ctx, dU, op->links, op->clover_dF, dres, psi, coefficient * op->csw);
+}
+}  // namespace lmx
+
 int lmx_wilson_pullback(lmx_wilson* op, void* dU, const void* dres, const void* psi, double coefficient) {
     clear_error();
     LMX_TRY(check_op(op));
     LMX_CHECK(dU && dres && psi, LMX_ERR_ARG, "bad pullback arguments");
-    return launch_pullback_f64(op->ctx, (double2*)dU, (const double2*)dres, (const double2*)psi, coefficient);
+    return operator_pullback(op, (double2*)dU, (const double2*)dres, (const double2*)psi, coefficient);
 }
 
 int lmx_ta_project(lmx_context* c, void* F, const void* links, const void* G) {
@@ -506,7 +518,6 @@ int lmx_wilson_force(lmx_wilson* op, void* G, void* F, const void* phi, void* et
     clear_error();
     LMX_TRY(check_op(op));
     LMX_CHECK(G && phi && eta && chi && work && params && result, LMX_ERR_ARG, "bad force arguments");
-    LMX_CHECK(op->csw == 0.0, LMX_ERR_UNSUPPORTED, "the force is implemented for the Wilson operator (csw == 0)");
     lmx_context* c = op->ctx;
     const size_t fb = 12 * (size_t)c->V;
     double2* w = (double2*)work;
@@ -514,11 +525,12 @@ int lmx_wilson_force(lmx_wilson* op, void* G, void* F, const void* phi, void* et
     LMX_HIP(hipMemsetAsync(eta, 0, fb * sizeof(double2), c->stream));
     LMX_TRY(lmx_cg_ddagd(op, eta, phi, r, p, Ap, tmp, params, result));
     Epilogue e{0, nullptr, nullptr, nullptr, nullptr};
-    LMX_TRY(launch_wilson_f64(c, op->links, nullptr, op->kappa, 0.0, (double2*)chi, (const double2*)eta, 1, 0, e));
+    LMX_TRY(launch_wilson_f64(c, op->links, op->clover64, op->kappa, op->csw, (double2*)chi, (const double2*)eta, 1, 0,
+                              e));
     // dresult = -2 chi (the dot pullback of g = -2 Re dot(chi, D eta))
     LMX_TRY(launch_scale(c, tmp, (const double2*)chi, -2.0, 0.0, (int64_t)fb));
     LMX_HIP(hipMemsetAsync(G, 0, 36 * (size_t)c->V * sizeof(double2), c->stream));
-    LMX_TRY(launch_pullback_f64(c, (double2*)G, tmp, (const double2*)eta, -op->kappa));
+    LMX_TRY(operator_pullback(op, (double2*)G, tmp, (const double2*)eta, -op->kappa));
     if (F) LMX_TRY(launch_ta_project(c, (double2*)F, op->links, (const double2*)G));
     LMX_HIP(hipStreamSynchronize(c->stream));
     return LMX_OK;

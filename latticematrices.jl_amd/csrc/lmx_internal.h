@@ -82,6 +82,7 @@ struct lmx_wilson {
     float2* clover32;
     __half2* clover16;
     bool lo_valid;
+    double2* clover_dF;  // scratch of the clover link pullback (6 planes), on first use
     // Autotuned launch shape per (precision, nrhs); 0 = untuned.
     int tuned_block[4][65];
 };
@@ -160,6 +161,9 @@ int launch_cg_init_state(lmx_context* ctx, CGState* st, double rtol, double atol
 // ---- pullback / TA (force.hip) ----
 int launch_pullback_f64(lmx_context* ctx, double2* dU, const double2* dres, const double2* psi, double coefficient);
 int launch_ta_project(lmx_context* ctx, double2* F, const double2* U, const double2* G);
+// clover part of the Wilson-clover link pullback (dF: 6*9*V scratch)
+int launch_clover_pullback(lmx_context* ctx, double2* dU, const double2* U, double2* dF, const double2* dres,
+                           const double2* psi, double coefficient);
 
 // ---- batched / mixed CG (mixed.hip) ----
 int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void* work, size_t bytes,

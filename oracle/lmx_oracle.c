@@ -39,6 +39,13 @@
  *                  gamma5-hermiticity, gauge covariance.
  *   pullback/TA  : central finite differences of S along su(3) directions,
  *                  linear-functional finite differences of <chi, D(U) psi>.
+ *   clover pullback / clover dSFdU : exact central differences of the
+ *                  functional Re <w, D_clover(U) psi> in single link entries
+ *                  (real-affine in any one entry, F recomputed from U),
+ *                  finite differences of S along su(3) directions, S = |eta|^2,
+ *                  planted mistakes (cSW sign, dropped clover part).  The
+ *                  SMatrix product order of the reference is not pinned: the
+ *                  products are restated as plain sums left to right.
  */
 #include <math.h>
 #include <stdint.h>
@@ -773,6 +780,254 @@ int orc_dsfdu(double* G, double* eta, double* chi, const double* U, const double
     for (long i = 0; i < 24 * V; ++i) dres[i] = -2.0 * chi[i];
     memset(G, 0, sizeof(double) * 4 * V * 18);
     orc_wilson_pullback(G, dres, eta, L, phases, -kappa);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Wilson-clover link pullback and force                                    */
+/* ------------------------------------------------------------------------ */
+
+/* gamma_mu gamma_nu per plane (clover_gamma_products,
+ * src/Operators/WilsonDiracCloverOperator.jl:10-13) from the chiral gamma
+ * matrices of src/Operators/Operators.jl:44-67; entries 0, +-1, +-i. */
+static cx gamma_products[6][4][4];
+static int gamma_ready = 0;
+static void init_gamma_products(void) {
+    if (gamma_ready) return;
+    cx gm[4][4][4];
+    memset(gm, 0, sizeof(gm));
+    /* gamma1 */
+    gm[0][0][3] = C(0, -1); gm[0][1][2] = C(0, -1); gm[0][2][1] = C(0, 1); gm[0][3][0] = C(0, 1);
+    /* gamma2 */
+    gm[1][0][3] = C(-1, 0); gm[1][1][2] = C(1, 0); gm[1][2][1] = C(1, 0); gm[1][3][0] = C(-1, 0);
+    /* gamma3 */
+    gm[2][0][2] = C(0, -1); gm[2][1][3] = C(0, 1); gm[2][2][0] = C(0, 1); gm[2][3][1] = C(0, -1);
+    /* gamma4 */
+    gm[3][0][2] = C(-1, 0); gm[3][1][3] = C(-1, 0); gm[3][2][0] = C(-1, 0); gm[3][3][1] = C(-1, 0);
+    for (int pl = 0; pl < 6; ++pl)
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                cx v = C(0, 0);
+                for (int k = 0; k < 4; ++k) v = cadd(v, cmul(gm[plane_mu[pl]][i][k], gm[plane_nu[pl]][k][j]));
+                gamma_products[pl][i][j] = v;
+            }
+    gamma_ready = 1;
+}
+
+/* _kernel_clover_field_cotangent_plane!, src/Operators/WilsonCloverPullback.jl:102-119:
+ * dF[plane][c][ic] += conj(coefficient) * sum_out dres[c,out] conj(sum_in g[out,in] psi[ic,in])
+ * (coefficient real).  dF layout as F: dF[((plane*V + site)*3+row)*3+col] complex. */
+static void clover_field_cotangent(double* dF, const double* dres, const double* psi, long V, double coefficient) {
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < V; ++s) {
+        cx d[12], p[12];
+        load_spinor(dres, s, d);
+        load_spinor(psi, s, p);
+        for (int pl = 0; pl < 6; ++pl)
+            for (int ic = 0; ic < 3; ++ic)
+                for (int c = 0; c < 3; ++c) {
+                    cx value = C(0, 0);
+                    for (int o = 0; o < 4; ++o) {
+                        cx tp = C(0, 0);
+                        for (int i = 0; i < 4; ++i) tp = cadd(tp, cmul(gamma_products[pl][o][i], p[i * 3 + ic]));
+                        value = cadd(value, cmul(d[o * 3 + c], cconj(tp)));
+                    }
+                    double* e = dF + (((long)pl * V + s) * 9 + c * 3 + ic) * 2;
+                    cx nv = cadd(C(e[0], e[1]), rmul(coefficient, value));
+                    e[0] = nv.re;
+                    e[1] = nv.im;
+                }
+    }
+}
+
+/* 3x3 product with plain complex arithmetic, sums left to right (the
+ * StaticArrays SMatrix products of WilsonCloverPullback.jl; their exact
+ * evaluation order is not pinned here). */
+static void mat_mul_plain(const cx* A, const cx* B, cx* out) {
+    cx r[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r[3 * i + j] = cadd(cadd(cmul(A[3 * i], B[j]), cmul(A[3 * i + 1], B[3 + j])), cmul(A[3 * i + 2], B[6 + j]));
+    memcpy(out, r, sizeof(r));
+}
+
+/* Path descriptors of the four leaves of plane (mu, nu)
+ * (_clover_path_descriptor, WilsonCloverPullback.jl:140-159): per factor the
+ * origin relative to the anchor, the direction, and whether it is adjointed. */
+typedef struct {
+    int origin[4][4];
+    int dir[4];
+    int dag[4];
+} leaf_path;
+static void make_path(const int* signed_dirs, leaf_path* lp) {
+    int pos[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        int sd = signed_dirs[k], d = (sd > 0 ? sd : -sd) - 1;
+        if (sd < 0) pos[d] -= 1;
+        memcpy(lp->origin[k], pos, sizeof(pos));
+        lp->dir[k] = d;
+        lp->dag[k] = sd < 0;
+        if (sd > 0) pos[d] += 1;
+    }
+}
+static long offset_site(const geom* g, long s, const int* off) {
+    for (int d = 0; d < 4; ++d) {
+        int n = off[d];
+        while (n > 0) { s = shift_site(g, s, d, +1); --n; }
+        while (n < 0) { s = shift_site(g, s, d, -1); ++n; }
+    }
+    return s;
+}
+
+/* _clover_path_occurrence_pullback (WilsonCloverPullback.jl:185-211) for
+ * occurrence K of the link at y: dQ = (dF - dF^dagger)/8 at the anchor;
+ * h = left^dagger dQ right^dagger, or right dQ^dagger left for an adjointed
+ * factor (left = M1..M_{K-1}, right = M_{K+1}..M4). */
+static void occurrence_pullback(const geom* g, const double* U, const double* dFp, long y, const leaf_path* lp, int K,
+                                cx h[9]) {
+    int neg[4];
+    for (int d = 0; d < 4; ++d) neg[d] = -lp->origin[K][d];
+    long anchor = offset_site(g, y, neg);
+    cx M[4][9];
+    for (int j = 0; j < 4; ++j) {
+        long site = offset_site(g, anchor, lp->origin[j]);
+        load_link(U, g, lp->dir[j], site, M[j]);
+        if (lp->dag[j]) mat_dag(M[j], M[j]);
+    }
+    cx dq[9], f[9];
+    const double* e = dFp + anchor * 18;
+    for (int k = 0; k < 9; ++k) f[k] = C(e[2 * k], e[2 * k + 1]);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) dq[3 * r + c] = rmul(0.125, csub(f[3 * r + c], cconj(f[3 * c + r])));
+    cx left[9], right[9], t[9];
+    int has_left = K > 0, has_right = K < 3;
+    if (K == 1) memcpy(left, M[0], sizeof(left));
+    if (K == 2) mat_mul_plain(M[0], M[1], left);
+    if (K == 3) {
+        mat_mul_plain(M[0], M[1], t);
+        mat_mul_plain(t, M[2], left);
+    }
+    if (K == 0) {
+        mat_mul_plain(M[1], M[2], t);
+        mat_mul_plain(t, M[3], right);
+    }
+    if (K == 1) mat_mul_plain(M[2], M[3], right);
+    if (K == 2) memcpy(right, M[3], sizeof(right));
+    if (lp->dag[K]) {
+        /* right * dQ^dagger * left */
+        cx dqd[9];
+        mat_dag(dq, dqd);
+        if (has_right) mat_mul_plain(right, dqd, t); else memcpy(t, dqd, sizeof(t));
+        if (has_left) mat_mul_plain(t, left, h); else memcpy(h, t, sizeof(t));
+    } else {
+        /* left^dagger * dQ * right^dagger */
+        cx ld[9], rd[9];
+        if (has_left) {
+            mat_dag(left, ld);
+            mat_mul_plain(ld, dq, t);
+        } else {
+            memcpy(t, dq, sizeof(t));
+        }
+        if (has_right) {
+            mat_dag(right, rd);
+            mat_mul_plain(t, rd, h);
+        } else {
+            memcpy(h, t, sizeof(t));
+        }
+    }
+}
+
+/* _kernel_clover_links_pullback! (WilsonCloverPullback.jl:236-280), planes in
+ * order (_clover_links_pullback!, :282-293): for each link site and plane
+ * (mu, nu), the four leaves (mu,nu,-mu,-nu), (nu,-mu,-nu,mu), (-mu,-nu,mu,nu),
+ * (-nu,mu,nu,-mu) and their four factors; dU[mu] += dmu, dU[nu] += dnu. */
+static void clover_links_pullback(double* dU, const double* U, const double* dF, const geom* g) {
+    leaf_path paths[6][4];
+    for (int pl = 0; pl < 6; ++pl) {
+        int m = plane_mu[pl] + 1, n = plane_nu[pl] + 1;
+        int p1[4] = {m, n, -m, -n}, p2[4] = {n, -m, -n, m}, p3[4] = {-m, -n, m, n}, p4[4] = {-n, m, n, -m};
+        make_path(p1, &paths[pl][0]);
+        make_path(p2, &paths[pl][1]);
+        make_path(p3, &paths[pl][2]);
+        make_path(p4, &paths[pl][3]);
+    }
+#pragma omp parallel for schedule(static)
+    for (long y = 0; y < g->V; ++y) {
+        for (int pl = 0; pl < 6; ++pl) {
+            const int mu = plane_mu[pl], nu = plane_nu[pl];
+            const double* dFp = dF + (long)pl * g->V * 18;
+            cx dmu[9], dnu[9];
+            for (int k = 0; k < 9; ++k) dmu[k] = dnu[k] = C(0, 0);
+            for (int p = 0; p < 4; ++p) {
+                cx hmu[9], hnu[9];
+                for (int k = 0; k < 9; ++k) hmu[k] = hnu[k] = C(0, 0);
+                for (int K = 0; K < 4; ++K) {
+                    cx h[9];
+                    occurrence_pullback(g, U, dFp, y, &paths[pl][p], K, h);
+                    cx* acc = paths[pl][p].dir[K] == mu ? hmu : hnu;
+                    for (int k = 0; k < 9; ++k) acc[k] = cadd(acc[k], h[k]);
+                }
+                for (int k = 0; k < 9; ++k) {
+                    dmu[k] = cadd(dmu[k], hmu[k]);
+                    dnu[k] = cadd(dnu[k], hnu[k]);
+                }
+            }
+            double* om = dU + ((long)mu * g->V + y) * 18;
+            double* on = dU + ((long)nu * g->V + y) * 18;
+            for (int k = 0; k < 9; ++k) {
+                cx a = cadd(C(om[2 * k], om[2 * k + 1]), dmu[k]);
+                om[2 * k] = a.re;
+                om[2 * k + 1] = a.im;
+                cx b = cadd(C(on[2 * k], on[2 * k + 1]), dnu[k]);
+                on[2 * k] = b.re;
+                on[2 * k + 1] = b.im;
+            }
+        }
+    }
+}
+
+/* wilson_clover_link_pullback! (WilsonCloverPullback.jl:349-389) with the
+ * coefficients of this library's pullback: dU += coefficient * (hopping
+ * pullback) + the clover part with coefficient * csw (the reference passes
+ * -kappa*scale and -kappa*cSW*scale).  dU: [4][V][3][3] complex. */
+void orc_clover_pullback(double* dU, const double* U, const double* dres, const double* psi, const int* L,
+                         const double* phases, double coefficient, double csw) {
+    geom g;
+    geom_init(&g, L, NULL);
+    init_gamma_products();
+    orc_wilson_pullback(dU, dres, psi, L, phases, coefficient);
+    double* dF = (double*)calloc((size_t)6 * g.V * 18, sizeof(double));
+    clover_field_cotangent(dF, dres, psi, g.V, coefficient * csw);
+    clover_links_pullback(dU, U, dF, &g);
+    free(dF);
+}
+
+/* dSFdU for the Wilson-clover operator: as orc_dsfdu with D the clover
+ * operator (F the field strength of U) and the clover link pullback. */
+int orc_dsfdu_clover(double* G, double* eta, double* chi, const double* U, const double* F, const double* phi,
+                     const int* L, const double* phases, double kappa, double csw, double rtol, int maxiter,
+                     int dot_order, double* work, orc_cg_result* res) {
+    geom g;
+    geom_init(&g, L, phases);
+    long V = g.V;
+    double *r = work, *p = work + 24 * V, *Ap = work + 48 * V, *tmp = work + 72 * V;
+    orc_op op;
+    memset(&op, 0, sizeof(op));
+    op.kind = ORC_OP_DDAGD_CLOVER;
+    op.U = U;
+    op.F = F;
+    memcpy(op.L, L, sizeof(op.L));
+    for (int k = 0; k < 8; ++k) op.phases[k] = phases ? phases[k] : (k % 2 ? 0.0 : 1.0);
+    op.kappa = kappa;
+    op.csw = csw;
+    op.tmp = tmp;
+    memset(eta, 0, sizeof(double) * 24 * V);
+    orc_cg(eta, phi, r, p, Ap, &op, V, rtol, 0.0, maxiter, dot_order, res);
+    orc_wilson_clover(chi, eta, U, F, L, phases, kappa, csw, 0);
+    double* dres = tmp;
+    for (long i = 0; i < 24 * V; ++i) dres[i] = -2.0 * chi[i];
+    memset(G, 0, sizeof(double) * 4 * V * 18);
+    orc_clover_pullback(G, U, dres, eta, L, phases, -kappa, csw);
     return 0;
 }
 

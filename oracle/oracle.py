@@ -74,6 +74,9 @@ def lib():
         L.orc_ta_force.argtypes = [P, P, P, ctypes.c_long]
         L.orc_dsfdu.argtypes = [P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_int, P, ctypes.POINTER(_OrcResult)]
+        L.orc_clover_pullback.argtypes = [P, P, P, P, P, P, ctypes.c_double, ctypes.c_double]
+        L.orc_dsfdu_clover.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(_OrcResult)]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -213,6 +216,39 @@ def wilson_pullback(dres, psi, L, coefficient, phases=(1, 1, 1, -1), dU=None):
     ph = _phases(phases)
     lib().orc_wilson_pullback(_ptr(dU), _ptr(dres), _ptr(psi), _dims(L), _ptr(ph), float(coefficient))
     return dU
+
+
+def clover_pullback(U, dres, psi, L, coefficient, csw, phases=(1, 1, 1, -1), dU=None):
+    """wilson_clover_link_pullback! with this library's coefficient convention:
+    dU += coefficient * hopping pullback + clover part with coefficient * csw."""
+    V = volume(L)
+    U = _c(U, (4, V, 3, 3))
+    dres = _c(dres, (V, 4, 3))
+    psi = _c(psi, (V, 4, 3))
+    dU = np.zeros((4, V, 3, 3), dtype=np.complex128) if dU is None else _c(dU, (4, V, 3, 3)).copy()
+    ph = _phases(phases)
+    lib().orc_clover_pullback(_ptr(dU), _ptr(U), _ptr(dres), _ptr(psi), _dims(L), _ptr(ph), float(coefficient),
+                              float(csw))
+    return dU
+
+
+def dsfdu_clover(U, phi, L, kappa, csw, phases=(1, 1, 1, -1), rtol=1e-12, maxiter=20000, order=1):
+    """dSFdU for the Wilson-clover operator: (G, eta, chi, CGResult)."""
+    V = volume(L)
+    U = _c(U, (4, V, 3, 3))
+    F = clover_field_strength(U, L)
+    phi = _c(phi, (V, 4, 3))
+    G = np.empty((4, V, 3, 3), dtype=np.complex128)
+    eta = np.empty((V, 4, 3), dtype=np.complex128)
+    chi = np.empty_like(eta)
+    work = np.empty((4, V, 4, 3), dtype=np.complex128)
+    ph = _phases(phases)
+    res = _OrcResult()
+    lib().orc_dsfdu_clover(_ptr(G), _ptr(eta), _ptr(chi), _ptr(U), _ptr(F), _ptr(phi), _dims(L), _ptr(ph),
+                           float(kappa), float(csw), float(rtol), int(maxiter), int(order), _ptr(work),
+                           ctypes.byref(res))
+    return G, eta, chi, CGResult(bool(res.converged), int(res.iterations), float(res.residual_norm),
+                                 float(res.relative_residual), REASONS[int(res.reason)])
 
 
 def ta_force(U, G):

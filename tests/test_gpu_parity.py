@@ -337,3 +337,39 @@ def test_apply_on_reference_storage(gpu, nw, phases, csw):
         assert np.all(C[halo] == sentinel[halo])
     with pytest.raises(lmx.ArgumentError):
         lmx.mul_padded_(psi_d, D, psi_d, nw)
+
+
+def test_clover_pullback_matches_oracle(gpu):
+    # wilson_clover_link_pullback! (hopping + clover field-strength chain rule)
+    L = (4, 2, 4, 6)
+    U = gen.links(L, "haar", 171)
+    psi = gen.gaussian_spinor(L, 172)
+    w = gen.gaussian_spinor(L, 173)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    D = lmx.WilsonDiracCloverOperator4D(lat.gauge_from_numpy(U), 0.12, 1.3)
+    dU = lmx.GaugeField(lat)
+    lmx.wilson_pullback_(dU, D, lat.spinor_from_numpy(w), lat.spinor_from_numpy(psi), -0.12)
+    ref = orc.clover_pullback(U, w, psi, L, -0.12, 1.3)
+    got = dU.to_numpy()
+    assert rel(got, ref) <= TOL
+    assert np.array_equal(got, ref), f"max rel {rel(got, ref):.3e}"
+
+
+def test_clover_force_matches_oracle(gpu):
+    L = (4, 4, 4, 4)
+    U = gen.links(L, "weak", 181, eps=0.3)
+    eta = gen.gaussian_spinor(L, 182)
+    F = orc.clover_field_strength(U, L)
+    phi = orc.wilson_clover(U, F, eta, L, 0.125, 1.0, dagger=True)
+    Go, Xo, chio, ro = orc.dsfdu_clover(U, phi, L, 0.125, 1.0, rtol=1e-10, order=1)
+    Fo = orc.ta_force(U, Go)
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracCloverOperator4D(Ug, 0.125, 1.0)
+    G, Fg = lmx.GaugeField(lat), lmx.GaugeField(lat)
+    res, eta_g, chi_g = lmx.dSFdU(G, D, lat.spinor_from_numpy(phi), force=Fg, rtol=1e-10)
+    assert res.converged and res.iterations == ro.iterations
+    assert np.array_equal(eta_g.to_numpy(), Xo)
+    assert np.array_equal(chi_g.to_numpy(), chio)
+    assert np.array_equal(G.to_numpy(), Go)
+    assert rel(Fg.to_numpy(), Fo) <= TOL
