@@ -115,6 +115,19 @@ __device__ __forceinline__ void pin_all(cx<R> (&a)[NR][12]) {
     for (int r = 0; r < NR; ++r) pin_acc(a[r]);
 }
 
+template <typename R>
+constexpr bool kSpatialOne = std::is_same<R, float>::value;
+
+// negate a half-spinor on a workgroup-uniform condition (a t wrap in the
+// streaming kernel): a scalar branch instead of per-lane selects
+template <typename R>
+__device__ __forceinline__ void neg_uniform(cx<R> h[6], bool neg) {
+    if (neg) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) h[k] = cneg(h[k]);
+    }
+}
+
 template <int MU, int PM, bool FWD, int NR, bool GP, class In, class Lk, typename R>
 __device__ __forceinline__ void hop_half(cx<R> (&acc)[NR][12], const In& in, const Lk& U, const Geom& g, int s,
                                          const int x[4], const int (&rr)[NR]) {
@@ -137,7 +150,9 @@ __device__ __forceinline__ void hop_half(cx<R> (&acc)[NR][12], const In& in, con
             }
         }
         half_project<PM, MU>(psi, h);
-        if (!GP) neg_select(h, ng);
+        // low-precision launches run with x, y, z phases exactly 1 (others take
+        // the general-phase kernel), so only t can wrap with a sign
+        if (!GP && (MU == 3 || !kSpatialOne<R>)) neg_select(h, ng);
         if (FWD)
             matvec_fwd(u, h, o);
         else
@@ -361,7 +376,12 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
                 if (w && !g.phase_one[3]) phase_spinor(psi, g.iph_re[3], g.iph_im[3]);
             }
             half_project<-FPM, 3>(psi, h);
-            if (!GP) neg_select(h, ng);
+            if (!GP) {
+                if constexpr (MODE == MODE_CLOVER)
+                    neg_select(h, ng);  // the clover variants sit at the register limit; keep their allocation
+                else
+                    neg_uniform(h, ng);
+            }
             matvec_bwd(u, h, B[r]);
         }
     }
@@ -422,7 +442,12 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
                     if (w && !g.phase_one[3]) phase_spinor(psi, g.ph_re[3], g.ph_im[3]);
                 }
                 half_project<FPM, 3>(psi, h);
-                if (!GP) neg_select(h, ng);
+                if (!GP) {
+                if constexpr (MODE == MODE_CLOVER)
+                    neg_select(h, ng);  // the clover variants sit at the register limit; keep their allocation
+                else
+                    neg_uniform(h, ng);
+            }
                 matvec_fwd(ut, h, o);
                 reconstruct_add_s<FPM, 3>(acc[r], o, c);
                 reconstruct_add<-FPM, 3>(acc[r], B[r]);
@@ -788,7 +813,9 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
     const Shape sh = pick_shape(ctx, ngroups, min_blocks, tl_hint);
     const int nb_site = nblocks(ctx->V) * ngroups;
     const int nb_stream = (int)(ctx->V / 256 / sh.TL) * ngroups;
-    const bool general = ctx->g.general != 0;
+    const Geom& gg = ctx->g;
+    const bool general = gg.general != 0 ||
+                         (kSpatialOne<typename In::R> && !(gg.phase_one[0] && gg.phase_one[1] && gg.phase_one[2]));
     hipStream_t st = ctx->stream;
     // FP64 streaming Wilson / hopping-only launches use the two-lanes-per-site kernel
     constexpr bool kTwoLane = LMX_TWO_LANE && std::is_same<In, SpF64In>::value && NR == 1;

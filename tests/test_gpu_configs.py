@@ -151,7 +151,10 @@ def test_clover_48x96_mixed_4rhs(gpu):
     D = lmx.WilsonDiracCloverOperator4D(Ug, kappa, csw)
     bg = lat.spinor_from_numpy(b)
     x = lat.spinor(4)
-    lmx.cg_batched_(x, D, bg, precision="h16", rtol=1e-3)  # build the low-precision copies, warm up
+    tune = lmx.autotune(D, "h16", 4)
+    blk = tune["rhs_block"]
+    print(f"  autotuned shape TL={blk >> 12} per={(blk >> 4) & 0xff} nr={blk & 15}", flush=True)
+    lmx.cg_batched_(x, D, bg, precision="h16", rtol=1e-3)  # warm up
     lmx.clear_matrix_(x)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
