@@ -115,7 +115,8 @@ int launch_wilson_padded(lmx_context* ctx, const double2* U, const double2* clov
 // whose block partials (one per 256 consecutive sites) go to row r of
 // partials (partials + r * pstride): 2 = |out|^2 on D; 3 = on D^dagger,
 // out = Ap is not stored but r -= alpha Ap, x += alpha p (p = aux; r in the
-// spinor format, x FP32) with |r|^2, skipping RHS whose done flag is set.
+// spinor format, x FP32) with |r|^2, skipping RHS whose done flag is set;
+// the x update can be deferred to the next iteration (apply_x, pending).
 struct LoArgs {
     int link_prec, spinor_prec;
     const void* links;
@@ -136,9 +137,16 @@ struct LoArgs {
     float* r_norm;
     void* x;  // FP32 component pairs, float4 [nrhs][6][V]
     const double* alpha;
+    const double* alpha_prev;
     int astride;
     const int* done;
+    const int* pending;
     int dstride;
+    // x update batched over two iterations: apply_x == 0 defers alpha p;
+    // apply_x == 1 adds alpha_prev pprev (where pending) and alpha p
+    int apply_x;
+    const void* pprev;
+    const float* pprev_norm;
 };
 int launch_wilson_lo(lmx_context* ctx, const LoArgs& a);
 

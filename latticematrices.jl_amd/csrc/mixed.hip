@@ -29,7 +29,10 @@ namespace lmx {
 
 struct MixRhs {
     double bnorm, tol, rr, rr_prev, pAp, alpha, beta, rref, rtrue;
+    double alpha_prev;  // alpha of the deferred x_inner += alpha p
     int done, converged, reason, since, iterations, updates;
+    int pending;        // a deferred alpha_prev p_prev awaits the next iteration (or a flush)
+    int pad;
 };
 struct MixBatch {
     int do_update;  // reliable update requested this iteration
@@ -48,7 +51,7 @@ struct InnerVec {
 
 struct MixPtrs {
     float4* xlo;  // inner solution: FP32 component pairs, float4 [nrhs][6][V]
-    InnerVec r, p, t;
+    InnerVec r, p[2], t;  // p double-buffered: iteration k reads p[k & 1] and writes p[(k + 1) & 1]
     double2* r64;
     double2* t64;
     double* partials;
@@ -82,7 +85,8 @@ static size_t layout(lmx_context* ctx, int nrhs, int sp, MixPtrs* P, char* base)
     MixPtrs q;
     q.xlo = (float4*)take(f32);
     q.r = inner();
-    q.p = inner();
+    q.p[0] = inner();
+    q.p[1] = inner();
     q.t = inner();
     q.r64 = (double2*)take(f64);
     q.t64 = (double2*)take(f64);
@@ -118,14 +122,23 @@ __device__ __forceinline__ void retire(MixRhs* st, MixBatch* batch, int reason, 
 
 enum { MX_INIT = 0, MX_PAP = 1, MX_RR = 2, MX_TRUE = 3 };
 
+// flag: MX_TRUE -- run even without a reliable-update request (final);
+//       MX_RR   -- this iteration deferred its x_inner += alpha p.
 template <int KIND>
 __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ partials, int nb, MixRhs* rhs,
-                                                MixBatch* batch, double rtol, double atol, int force) {
+                                                MixBatch* batch, double rtol, double atol, int flag) {
     const int i = blockIdx.x;
     MixRhs* st = rhs + i;
     if (KIND == MX_PAP && i == 0 && threadIdx.x == 0) batch->do_update = 0;
-    if (KIND == MX_TRUE && !force && !batch->do_update) return;
+    if (KIND == MX_TRUE && !flag && !batch->do_update) return;
     if (st->done) return;
+    if (threadIdx.x == 0) {
+        if (KIND == MX_RR) {
+            st->pending = flag;
+            if (flag) st->alpha_prev = st->alpha;
+        }
+        if (KIND == MX_TRUE) st->pending = 0;  // k_mx_accum folded the deferred term
+    }
     __shared__ double lds4[4];
     const double A = finish_partials(partials + (size_t)(2 * i) * nb, nb, lds4);
     const double B = (KIND == MX_INIT) ? finish_partials(partials + (size_t)(2 * i + 1) * nb, nb, lds4) : 0.0;
@@ -227,21 +240,33 @@ __global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta,
     }
 }
 
-// reliable update, part 1: x += x_lo (FP64), x_lo = 0
-__global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float4* __restrict__ xlo, int V,
+// reliable update, part 1: x += x_lo (+ the deferred alpha p) in FP64, x_lo = 0
+template <class SI>
+__global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float4* __restrict__ xlo, SI pcur, int V,
                                                   const MixRhs* rhs, const MixBatch* batch, int force) {
     const int i = blockIdx.y;
     if ((!force && !batch->do_update) || rhs[i].done) return;
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= V) return;
     const size_t o = (size_t)i * 12 * V, o4 = (size_t)i * 6 * V;
+    cx<float> q[12];
+    const bool pend = rhs[i].pending != 0;  // this iteration deferred alpha p (p not yet updated)
+    const double ap = rhs[i].alpha_prev;
+    if (pend) pcur.ld12(i, s, q);
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
         const float4 l = xlo[o4 + (size_t)m * V + s];
         const size_t e0 = o + (size_t)(2 * m) * V + s, e1 = e0 + V;
+        double a0 = (double)l.x, b0 = (double)l.y, a1 = (double)l.z, b1 = (double)l.w;
+        if (pend) {
+            a0 += ap * (double)q[2 * m].re;
+            b0 += ap * (double)q[2 * m].im;
+            a1 += ap * (double)q[2 * m + 1].re;
+            b1 += ap * (double)q[2 * m + 1].im;
+        }
         const double2 h0 = x[e0], h1 = x[e1];
-        x[e0] = make_double2(h0.x + (double)l.x, h0.y + (double)l.y);
-        x[e1] = make_double2(h1.x + (double)l.z, h1.y + (double)l.w);
+        x[e0] = make_double2(h0.x + a0, h0.y + b0);
+        x[e1] = make_double2(h1.x + a1, h1.y + b1);
         xlo[o4 + (size_t)m * V + s] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
@@ -334,18 +359,23 @@ static LoArgs lo_args(lmx_wilson* op, int link_prec, int spinor_prec) {
 // One inner iteration's operator work, in two launches with fused epilogues:
 //   t = D p, with the block partials of |t|^2 = p.Ap (FP64 sums), then
 //   (after the finalize has set alpha) Ap = D^dagger t, kept in registers:
-//   r -= alpha Ap, x_inner += alpha p, with the block partials of |r|^2.
+//   r -= alpha Ap with the block partials of |r|^2, and the x_inner update
+//   batched over two iterations: an even iteration defers alpha p (its p
+//   stays in its buffer, the finalize marks it pending), the odd iteration
+//   adds alpha_prev p_prev + alpha p -- the x_inner traffic of one iteration
+//   for two.  A reliable update after an even iteration folds the pending
+//   term into x in k_mx_accum.
 static_assert(sizeof(MixRhs) % sizeof(double) == 0, "MixRhs is addressed in doubles and ints");
 
-static int apply_d_norm(MixRun& R) {
+static int apply_d_norm(MixRun& R, int e) {
     LoArgs a = lo_args(R.op, R.link_prec, R.spinor_prec);
     a.nrhs = R.nrhs;
     a.rhs_block = R.rhs_block;
     a.gate = &R.w.batch->n_active;
     a.out = R.w.t.p;
     a.out_norm = R.w.t.n;
-    a.in = R.w.p.p;
-    a.in_norm = R.w.p.n;
+    a.in = R.w.p[e].p;
+    a.in_norm = R.w.p[e].n;
     a.flags = 0;
     a.epi = 2;
     a.partials = R.w.partials;
@@ -353,7 +383,7 @@ static int apply_d_norm(MixRun& R) {
     return launch_wilson_lo(R.ctx, a);
 }
 
-static int apply_ddag_xr(MixRun& R) {
+static int apply_ddag_xr(MixRun& R, int e) {
     LoArgs a = lo_args(R.op, R.link_prec, R.spinor_prec);
     a.nrhs = R.nrhs;
     a.rhs_block = R.rhs_block;
@@ -361,16 +391,21 @@ static int apply_ddag_xr(MixRun& R) {
     a.out = nullptr;  // Ap is consumed in registers
     a.in = R.w.t.p;
     a.in_norm = R.w.t.n;
-    a.aux = R.w.p.p;
-    a.aux_norm = R.w.p.n;
+    a.aux = R.w.p[e].p;
+    a.aux_norm = R.w.p[e].n;
+    a.pprev = R.w.p[1 - e].p;
+    a.pprev_norm = R.w.p[1 - e].n;
+    a.apply_x = e;
     a.flags = LMX_DAGGER;
     a.epi = 3;
     a.r = R.w.r.p;
     a.r_norm = R.w.r.n;
     a.x = R.w.xlo;
     a.alpha = (const double*)((const char*)R.w.rhs + offsetof(MixRhs, alpha));
+    a.alpha_prev = (const double*)((const char*)R.w.rhs + offsetof(MixRhs, alpha_prev));
     a.astride = (int)(sizeof(MixRhs) / sizeof(double));
     a.done = (const int*)((const char*)R.w.rhs + offsetof(MixRhs, done));
+    a.pending = (const int*)((const char*)R.w.rhs + offsetof(MixRhs, pending));
     a.dstride = (int)(sizeof(MixRhs) / sizeof(int));
     a.partials = R.w.partials;
     a.pstride = 2L * nblocks(R.ctx->V);
@@ -383,32 +418,44 @@ static void enqueue_true_res(MixRun& R, dim3 grid, int force) {
     hipLaunchKernelGGL((k_mx_true_res<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V), V,
                        R.w.partials, R.w.rhs, R.w.batch, force);
 }
+// p[1 - e] = r + beta p[e]
 template <class SI, class SO>
-static void enqueue_update_p(MixRun& R, dim3 grid) {
+static void enqueue_update_p(MixRun& R, dim3 grid, int e) {
     const int V = (int)R.ctx->V;
     hipLaunchKernelGGL((k_mx_update_p<SI, SO>), grid, dim3(256), 0, R.ctx->stream, in_of<SI>(R.w.r, V),
-                       in_of<SI>(R.w.p, V), out_of<SO>(R.w.p, V), V, R.w.rhs);
+                       in_of<SI>(R.w.p[e], V), out_of<SO>(R.w.p[1 - e], V), V, R.w.rhs);
 }
 template <class SO>
 static void enqueue_init(MixRun& R, dim3 grid) {
     const int V = (int)R.ctx->V;
     hipLaunchKernelGGL((k_mx_init<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V),
-                       out_of<SO>(R.w.p, V), R.w.xlo, V, R.w.partials);
+                       out_of<SO>(R.w.p[0], V), R.w.xlo, V, R.w.partials);
+}
+// reliable-update part 1 (gated, or forced at the end): x += x_inner (+ pending alpha p[e])
+static void enqueue_accum(MixRun& R, dim3 grid, int e, int force) {
+    const int V = (int)R.ctx->V;
+    if (R.spinor_prec == LMX_H16)
+        hipLaunchKernelGGL((k_mx_accum<SpH16In>), grid, dim3(256), 0, R.ctx->stream, R.x, R.w.xlo,
+                           in_of<SpH16In>(R.w.p[e], V), V, R.w.rhs, R.w.batch, force);
+    else
+        hipLaunchKernelGGL((k_mx_accum<SpF32In>), grid, dim3(256), 0, R.ctx->stream, R.x, R.w.xlo,
+                           in_of<SpF32In>(R.w.p[e], V), V, R.w.rhs, R.w.batch, force);
 }
 
-static int enqueue_iteration(MixRun& R) {
+// One iteration with parity e (iteration k has e = k & 1).
+static int enqueue_iteration(MixRun& R, int e) {
     lmx_context* c = R.ctx;
-    const int V = (int)c->V;
     const bool h16 = R.spinor_prec == LMX_H16;
     dim3 grid(nblocks(c->V), R.nrhs), blk(256);
     const int nb = nblocks(c->V);
     MixRhs* rhs = R.w.rhs;
     MixBatch* bt = R.w.batch;
-    LMX_TRY(apply_d_norm(R));
+    LMX_TRY(apply_d_norm(R, e));
     hipLaunchKernelGGL(k_mx_fin<MX_PAP>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
-    LMX_TRY(apply_ddag_xr(R));
-    hipLaunchKernelGGL(k_mx_fin<MX_RR>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
-    hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, R.x, R.w.xlo, V, rhs, bt, 0);
+    LMX_TRY(apply_ddag_xr(R, e));
+    hipLaunchKernelGGL(k_mx_fin<MX_RR>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0,
+                       e == 0 ? 1 : 0);
+    enqueue_accum(R, grid, e, 0);
     LMX_TRY(apply_ddagd64(R, R.w.r64, R.x, R.w.t64, &bt->do_update));
     if (h16)
         enqueue_true_res<SpH16Out>(R, grid, 0);
@@ -416,9 +463,9 @@ static int enqueue_iteration(MixRun& R) {
         enqueue_true_res<SpF32Out>(R, grid, 0);
     hipLaunchKernelGGL(k_mx_fin<MX_TRUE>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
     if (h16)
-        enqueue_update_p<SpH16In, SpH16Out>(R, grid);
+        enqueue_update_p<SpH16In, SpH16Out>(R, grid, e);
     else
-        enqueue_update_p<SpF32In, SpF32Out>(R, grid);
+        enqueue_update_p<SpF32In, SpF32Out>(R, grid, e);
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }
@@ -489,7 +536,10 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
                        R.w.batch, params->rtol, params->atol, 0);
     LMX_HIP(hipGetLastError());
 
-    const int chunk = params->check_every > 0 ? params->check_every : 10;
+    // iterations between host polls; even with a graph, so every replay
+    // starts at an even iteration (the x_inner batching pairs iterations)
+    int chunk = params->check_every > 0 ? params->check_every : 10;
+    if (params->use_graph && (chunk & 1)) ++chunk;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     if (params->use_graph) {
@@ -500,7 +550,7 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
         int rc = LMX_OK;
         if (e == hipSuccess) {
-            for (int k = 0; k < chunk && rc == LMX_OK; ++k) rc = enqueue_iteration(R);
+            for (int k = 0; k < chunk && rc == LMX_OK; ++k) rc = enqueue_iteration(R, k & 1);
             e = hipStreamEndCapture(cap, &graph);
         }
         c->stream = saved;
@@ -526,7 +576,7 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
             launched += chunk;
         } else {
             const int n = std::min(chunk, params->maxiter - launched);
-            for (int k = 0; k < n && rc == LMX_OK; ++k) rc = enqueue_iteration(R);
+            for (int k = 0; k < n && rc == LMX_OK; ++k) rc = enqueue_iteration(R, (launched + k) & 1);
             if (rc != LMX_OK) break;
             launched += n;
         }
@@ -535,7 +585,8 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     if (graph) (void)hipGraphDestroy(graph);
     LMX_TRY(rc);
     // RHS stopped by maxiter: fold the inner solution in and report the true residual
-    hipLaunchKernelGGL(k_mx_accum, grid, blk, 0, c->stream, x, R.w.xlo, V, R.w.rhs, R.w.batch, 1);
+    // the last iteration's p buffer holds a still pending alpha p
+    enqueue_accum(R, grid, launched > 0 ? ((launched - 1) & 1) : 0, 1);
     LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
     if (R.spinor_prec == LMX_H16)
         enqueue_true_res<SpH16Out>(R, grid, 1);
@@ -665,9 +716,12 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     lx.r_norm = norm_of(rbuf);
     lx.x = xbuf;
     lx.alpha = alpha;
+    lx.alpha_prev = alpha;
     lx.astride = 1;
     lx.done = done;
+    lx.pending = done;  // zeros: no deferred term
     lx.dstride = 1;
+    lx.apply_x = 1;     // the x-updating (odd) iteration
     lx.partials = parts;
     lx.pstride = 2L * (long)nbk;
     auto pair = [&]() {

@@ -188,17 +188,24 @@ constexpr int kMaxTL = 32;  // t-slices per workgroup at most (LDS of the fused 
 enum { EPI_NONE = 0, EPI_PAP = 1, EPI_NORM = 2, EPI_XR = 3 };
 
 struct NoEx {};
-// Extra fields of EPI_XR, all offset to the launch's first RHS.
+// Extra fields of EPI_XR, all offset to the launch's first RHS.  The x
+// update is batched over two iterations: with apply_x == 0 the kernel leaves
+// x alone (alpha p is deferred, p kept in its buffer); with apply_x == 1 it
+// adds the deferred alpha_prev p_prev (where pending) and alpha p.
 template <class In, class Out>
 struct XrEx {
     In rin;        // r (read) ...
     Out rout;      // ... and written in place
     SpX4In xin;    // inner solution x (FP32, component pairs)
     SpX4Out xout;
-    const double* alpha;  // alpha of RHS r at alpha[r * astride]
+    In pprev;      // the previous iteration's p (apply_x == 1)
+    const double* alpha;       // alpha of RHS r at alpha[r * astride]
+    const double* alpha_prev;  // deferred alpha at alpha_prev[r * astride]
     int astride;
-    const int* done;      // retired RHS (no update) at done[r * dstride]
+    const int* done;           // retired RHS (no update) at done[r * dstride]
+    const int* pending;        // a deferred alpha p is pending at pending[r * dstride]
     int dstride;
+    int apply_x;
 };
 
 // Finish one site from its own spinor psi: out = psi - kappa*hop (+ clover),
@@ -225,21 +232,32 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
     double red = 0.0;
     if constexpr (EPI == EPI_XR) {
         const R a = R(ex.alpha[(long)r * ex.astride]);
-        cx<R> rv[12], pv[12], xv[12];
+        cx<R> rv[12];
         ex.rin.ld12(r, s, rv);
-        aux.ld12(r, s, pv);
-        ex.xin.ld12(r, s, xv);
         R site = R(0);  // per-site sum in working precision, accumulated over sites in FP64
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
             rv[k] = mk<R>(fmaR(-a, res[k].re, rv[k].re), fmaR(-a, res[k].im, rv[k].im));
-            xv[k] = mk<R>(fmaR(a, pv[k].re, xv[k].re), fmaR(a, pv[k].im, xv[k].im));
             site = fmaR(rv[k].re, rv[k].re, site);
             site = fmaR(rv[k].im, rv[k].im, site);
         }
         red = double(site);
-        if (store && ex.done[(long)r * ex.dstride] == 0) {
-            ex.rout.st12(r, s, rv);
+        const bool live = store && ex.done[(long)r * ex.dstride] == 0;
+        if (live) ex.rout.st12(r, s, rv);
+        if (live && ex.apply_x) {
+            cx<R> pv[12], xv[12];
+            aux.ld12(r, s, pv);
+            ex.xin.ld12(r, s, xv);
+            if (ex.pending[(long)r * ex.dstride]) {
+                const R ap = R(ex.alpha_prev[(long)r * ex.astride]);
+                cx<R> qv[12];
+                ex.pprev.ld12(r, s, qv);
+#pragma unroll
+                for (int k = 0; k < 12; ++k)
+                    xv[k] = mk<R>(fmaR(ap, qv[k].re, xv[k].re), fmaR(ap, qv[k].im, xv[k].im));
+            }
+#pragma unroll
+            for (int k = 0; k < 12; ++k) xv[k] = mk<R>(fmaR(a, pv[k].re, xv[k].re), fmaR(a, pv[k].im, xv[k].im));
             ex.xout.st12(r, s, xv);
         }
         return red;
@@ -903,9 +921,18 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
         const NoEx none{};
         if (a.epi == EPI_XR) {
             In p = In::at(a.aux, a.aux_norm, V, r0);
-            XrEx<In, Out> ex{In::at(a.r, a.r_norm, V, r0), Out::at(a.r, a.r_norm, V, r0),
-                             SpX4In::at(a.x, nullptr, V, r0), SpX4Out::at(a.x, nullptr, V, r0),
-                             a.alpha + (long)r0 * a.astride, a.astride, a.done + (long)r0 * a.dstride, a.dstride};
+            XrEx<In, Out> ex{In::at(a.r, a.r_norm, V, r0),
+                             Out::at(a.r, a.r_norm, V, r0),
+                             SpX4In::at(a.x, nullptr, V, r0),
+                             SpX4Out::at(a.x, nullptr, V, r0),
+                             In::at(a.pprev ? a.pprev : a.aux, a.pprev ? a.pprev_norm : a.aux_norm, V, r0),
+                             a.alpha + (long)r0 * a.astride,
+                             a.alpha_prev + (long)r0 * a.astride,
+                             a.astride,
+                             a.done + (long)r0 * a.dstride,
+                             a.pending + (long)r0 * a.dstride,
+                             a.dstride,
+                             a.apply_x};
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, XrEx<In, Out>, NR, EPI_XR, false, false>(
                 ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl)));
         } else if (a.epi == EPI_NORM) {
