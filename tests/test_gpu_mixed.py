@@ -142,3 +142,57 @@ def test_mixed_stream_shape(gpu, monkeypatch, precision):
         print(f"  {precision} iters={r.iterations} updates={r.reliable_updates} true_rel={tr:.3e} err={err:.3e}",
               flush=True)
         assert r.converged and tr <= rtol * 1.05 and err <= 1e-7
+
+
+def test_many_rhs_split_over_launches_h16(gpu):
+    # 14 RHS in launches of 6, 6, 2 at 2 per work item, against one launch of
+    # 14 at 1 per work item: identical iterates (each RHS is computed and
+    # reduced independently of the grouping), all converged to the true residual
+    L, kappa, rtol = (8, 8, 4, 8), 0.124, 1e-10
+    U, b, lat, Ug, D = setup(L, "weak", 370, kappa, nrhs=14)
+    bg = lat.spinor_from_numpy(b)
+    x1, x2 = lat.spinor(14), lat.spinor(14)
+    r1 = lmx.cg_batched_(x1, D, bg, precision="h16", rtol=rtol, use_graph=True, rhs_block=(6 << 4) | 2)
+    r2 = lmx.cg_batched_(x2, D, bg, precision="h16", rtol=rtol, use_graph=False, rhs_block=(14 << 4) | 1)
+    assert [r.iterations for r in r1] == [r.iterations for r in r2]
+    assert np.array_equal(x1.to_numpy(), x2.to_numpy())
+    checks = oracle_check(U, L, kappa, b[:3], x1.to_numpy()[:3], rtol)
+    for r, (tr, err) in zip(r1, checks):
+        assert r.converged and tr <= rtol * 1.05 and err <= 1e-7
+    assert all(r.converged for r in r1)
+
+
+@pytest.mark.parametrize("check_every", [1, 3, 7])
+def test_batched_x_update_any_poll_interval(gpu, check_every):
+    # the x update pairs iterations (even ones defer alpha p); odd poll
+    # intervals, graph (rounded up to even) or eager, must not change the iterates
+    L, kappa = (8, 4, 4, 8), 0.12
+    U, b, lat, Ug, D = setup(L, "weak", 380, kappa, nrhs=2)
+    bg = lat.spinor_from_numpy(b)
+    xs = []
+    for graph in (False, True):
+        x = lat.spinor(2)
+        res = lmx.cg_batched_(x, D, bg, precision="h16", use_graph=graph, check_every=check_every)
+        assert all(r.converged for r in res)
+        xs.append((x.to_numpy(), [r.iterations for r in res]))
+    assert xs[0][1] == xs[1][1]
+    assert np.array_equal(xs[0][0], xs[1][0])
+    # stopped by maxiter after an even (deferring) iteration: the pending term is folded in
+    x = lat.spinor(2)
+    res = lmx.cg_batched_(x, D, bg, precision="h16", maxiter=5, check_every=1)
+    assert res[0].reason == "maximum_iterations"
+    A = orc.NormalOp(orc.OP_DDAGD_WILSON, L, U=U, kappa=kappa)
+    tr = np.linalg.norm(A.apply(x.to_numpy()[0]) - b[0]) / np.linalg.norm(b[0])
+    assert abs(tr - res[0].true_relative_residual) <= 1e-6 * tr
+
+
+def test_rhs_count_limits(gpu):
+    L, kappa = (4, 4, 4, 8), 0.12
+    U, b, lat, Ug, D = setup(L, "weak", 390, kappa, nrhs=64)
+    bg = lat.spinor_from_numpy(b)
+    x = lat.spinor(64)
+    res = lmx.cg_batched_(x, D, bg, precision="h16", use_graph=True)
+    assert len(res) == 64 and all(r.converged for r in res)
+    b65 = lat.spinor(65)
+    with pytest.raises(lmx.ArgumentError):
+        lmx.cg_batched_(lat.spinor(65), D, b65, precision="h16")
