@@ -263,7 +263,7 @@ def main():
                          "frac": ach_dm / HBM_PEAK_GBS},
         "roofline": {"bound": "hbm", "achieved": ach_d64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_d64 / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stream<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, -1, 0, 1, 0, false> "
+                     "kernel": "k_stream<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, NoEx, -1, 0, 1, 0, false> "
                                "(FP64 D_W, 32^4)",
                      "bytes_per_launch": bytes_d64, "us_per_launch": t_d64 * 1e3,
                      "traffic_source": traffic_src},
