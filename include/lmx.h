@@ -272,6 +272,80 @@ typedef struct {
  * use it for later lmx_cg_batched calls with rhs_block = 0. */
 int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char* cache_path, lmx_tune_result* out);
 
+/* ======================================================================= */
+/* Site random numbers and lattice fills                                    */
+/* ======================================================================= */
+
+/* RNGStreamKey(seed, sweep, direction, color, subgroup)
+ * (src/utilities/randomgenerator.jl:98-118): the tags of one stream family;
+ * each zero-based global site id (x fastest, global_site_id,
+ * src/LatticeMatrices.jl:334-345) owns one stream of the family. */
+typedef struct {
+    uint64_t seed;
+    uint64_t sweep;
+    uint32_t direction;
+    uint32_t color;
+    uint32_t subgroup;
+    uint32_t reserved;
+} lmx_rng_key;
+
+/* Site RNG algorithms: PCG32(), Xoshiro256PlusPlus(), Philox4x32()
+ * (randomgenerator.jl:52-59). */
+enum { LMX_RNG_PCG32 = 0, LMX_RNG_XOSHIRO256PP = 1, LMX_RNG_PHILOX4X32 = 2 };
+/* Element types of a random fill: Float32, Float64, ComplexF32, ComplexF64
+ * (_check_randomize_eltype, src/LinearAlgebras/linearalgebra_D.jl:2546-2549). */
+enum { LMX_ELEM_F32 = 0, LMX_ELEM_F64 = 1, LMX_ELEM_C64 = 2, LMX_ELEM_C128 = 3 };
+/* randomize_matrix! (uniform in [-0.5, 0.5)) or randomize_gaussian_matrix!
+ * (normal, standard deviation sigma). */
+enum { LMX_FILL_UNIFORM = 0, LMX_FILL_GAUSSIAN = 1 };
+
+/* Draws per site for lmx_site_rng_draw, taken in this order. */
+typedef struct {
+    int32_t words;     /* rand_u32 */
+    int32_t uniform32; /* rand_uniform(rng, Float32) */
+    int32_t uniform64; /* rand_uniform(rng, Float64) */
+    int32_t normal32;  /* rand_normal_pair(rng, Float32), two values per pair (even count) */
+    int32_t normal64;  /* rand_normal_pair(rng, Float64), two values per pair (even count) */
+    int32_t bounded;   /* rand_bounded(rng, upper) */
+    uint32_t upper;    /* nonzero when bounded > 0 */
+    int32_t reserved;
+} lmx_rng_draws;
+
+/* For each site i in [0, nsites): stream = site_rng(key, first_site + i,
+ * algorithm) (randomgenerator.jl:152-200), then the draws of *n in order
+ * (rand_u32 :219-319, rand_uniform :341-352, rand_normal_pair :380-386,
+ * rand_bounded :406-414).  Outputs are device arrays [count, nsites]
+ * column-major (the draws of one site are contiguous).  Integer and uniform
+ * outputs are bit-identical to the reference; normals agree to the last bits
+ * of the device log / sqrt / sincos. */
+int lmx_site_rng_draw(lmx_context* ctx, const lmx_rng_key* key, int algorithm, uint64_t first_site, int32_t nsites,
+                      const lmx_rng_draws* n, uint32_t* words, float* uniform32, double* uniform64, float* normal32,
+                      double* normal64, uint32_t* bounded);
+
+/* randomize_matrix!(C, key; rng_algorithm) (dist = LMX_FILL_UNIFORM,
+ * src/LinearAlgebras/linearalgebra_D.jl:2560-2582) and
+ * randomize_gaussian_matrix!(C, key; sigma, rng_algorithm) (LMX_FILL_GAUSSIAN,
+ * :2610-2637) on a LatticeMatrix array in the reference's own storage: C is
+ * A[nc1, nc2, L1+2nw, ..., L4+2nw] column-major of element type elem over the
+ * context's lattice.  global_offset / global_dims place that lattice as a box
+ * of a larger global lattice (both NULL: it is the whole lattice).  When it is
+ * the whole lattice the halo is then refilled as set_halo! does
+ * (src/LatticeMatrices_core.jl:707-723, the context's boundary phases: real
+ * for real fields, +-1 for single precision); a box of a larger lattice has
+ * only its core written.  sigma >= 0 (ArgumentError otherwise). */
+int lmx_randomize_padded(lmx_context* ctx, void* C, int elem, int nc1, int nc2, int nw, int dist,
+                         const lmx_rng_key* key, double sigma, int algorithm, const int32_t* global_offset,
+                         const int32_t* global_dims);
+
+/* The same fill of nrhs (3,4) ComplexF64 spinor fields written straight into
+ * the library's F64 spinor layout (field r from keys[r]): identical to
+ * randomize_*_matrix!(psi, keys[r]) followed by lmx_spinor_import.  The
+ * D^dagger D CG benchmark's source is keys = {RNGStreamKey(seed + 1)},
+ * LMX_FILL_GAUSSIAN, sigma = sqrt(1/2), Philox
+ * (benchmark/ddagd_cg/ddagd_cg.jl:773-774). */
+int lmx_spinor_randomize(lmx_context* ctx, void* dst, int nrhs, int dist, const lmx_rng_key* keys, double sigma,
+                         int algorithm, const int32_t* global_offset, const int32_t* global_dims);
+
 #ifdef __cplusplus
 }
 #endif

@@ -259,6 +259,27 @@ int lmx_spinor_export(lmx_context* c, void* dst, int nw, const void* src, int nr
     LMX_CHECK(dst && src && nrhs >= 0, LMX_ERR_ARG, "bad spinor_export arguments");
     return spinor_export(c, dst, nw, src, nrhs);
 }
+int lmx_site_rng_draw(lmx_context* c, const lmx_rng_key* key, int algorithm, uint64_t first_site, int32_t nsites,
+                      const lmx_rng_draws* n, uint32_t* words, float* uniform32, double* uniform64, float* normal32,
+                      double* normal64, uint32_t* bounded) {
+    clear_error();
+    LMX_TRY(check_ctx(c));
+    return launch_site_rng_draw(c, key, algorithm, first_site, nsites, n, words, uniform32, uniform64, normal32,
+                                normal64, bounded);
+}
+int lmx_randomize_padded(lmx_context* c, void* C, int elem, int nc1, int nc2, int nw, int dist,
+                         const lmx_rng_key* key, double sigma, int algorithm, const int32_t* global_offset,
+                         const int32_t* global_dims) {
+    clear_error();
+    LMX_TRY(check_ctx(c));
+    return launch_randomize_padded(c, C, elem, nc1, nc2, nw, dist, key, sigma, algorithm, global_offset, global_dims);
+}
+int lmx_spinor_randomize(lmx_context* c, void* dst, int nrhs, int dist, const lmx_rng_key* keys, double sigma,
+                         int algorithm, const int32_t* global_offset, const int32_t* global_dims) {
+    clear_error();
+    LMX_TRY(check_ctx(c));
+    return launch_spinor_randomize(c, dst, nrhs, dist, keys, sigma, algorithm, global_offset, global_dims);
+}
 int lmx_links_import(lmx_context* c, void* dst, const void* const src[4], int nw) {
     clear_error();
     LMX_TRY(check_ctx(c));

@@ -170,6 +170,14 @@ int launch_cg_init_state(lmx_context* ctx, CGState* st, double rtol, double atol
 int launch_pullback_f64(lmx_context* ctx, double2* dU, const double2* dres, const double2* psi, double coefficient);
 int launch_ta_project(lmx_context* ctx, double2* F, const double2* U, const double2* G);
 // clover part of the Wilson-clover link pullback (dF: 6*9*V scratch)
+int launch_site_rng_draw(lmx_context* ctx, const lmx_rng_key* key, int algorithm, uint64_t first_site, int nsites,
+                         const lmx_rng_draws* n, uint32_t* words, float* u32, double* u64, float* n32, double* n64,
+                         uint32_t* bnd);
+int launch_randomize_padded(lmx_context* ctx, void* C, int elem, int nc1, int nc2, int nw, int dist,
+                            const lmx_rng_key* key, double sigma, int algorithm, const int32_t* goff,
+                            const int32_t* gdims);
+int launch_spinor_randomize(lmx_context* ctx, void* dst, int nrhs, int dist, const lmx_rng_key* keys, double sigma,
+                            int algorithm, const int32_t* goff, const int32_t* gdims);
 int launch_clover_pullback(lmx_context* ctx, double2* dU, const double2* U, double2* dF, const double2* dres,
                            const double2* psi, double coefficient);
 

@@ -10,10 +10,13 @@ from .lattice import (Adjoint, CGResult, DdagDOp, GaugeField, Lattice, ScaledOpe
                       WilsonDiracCloverOperator4D, WilsonDiracOperator4D, WilsonDiracOperator4D_Donly, adjoint,
                       axpby_, cg_, clear_matrix_, dot, dSFdU, mul_, mul_padded_, pseudofermion_action, solve_, substitute_,
                       ta_project_, wilson_pullback_, cg_batched_, autotune)
+from .rng import (DEFAULT_RANDOMIZE_SEED, RNGStreamKey, randomize_gaussian_matrix_, randomize_matrix_,
+                  site_rng_draw)
 
 __all__ = [
     "ArgumentError", "LmxError", "Adjoint", "CGResult", "DdagDOp", "GaugeField", "Lattice", "ScaledOperator",
     "SpinorField", "WilsonDiracCloverOperator4D", "WilsonDiracOperator4D", "WilsonDiracOperator4D_Donly",
     "adjoint", "axpby_", "cg_", "clear_matrix_", "dot", "dSFdU", "mul_", "mul_padded_", "pseudofermion_action", "solve_",
     "substitute_", "ta_project_", "wilson_pullback_", "generators", "cg_batched_", "autotune",
+    "DEFAULT_RANDOMIZE_SEED", "RNGStreamKey", "randomize_gaussian_matrix_", "randomize_matrix_", "site_rng_draw",
 ]

@@ -17,6 +17,9 @@ LMX_F64, LMX_F32, LMX_F16, LMX_H16 = 0, 1, 2, 3
 LMX_DAGGER, LMX_DONLY = 1, 2
 LMX_CG_CONVERGED, LMX_CG_MAXITER, LMX_CG_NONPOSITIVE_CURVATURE, LMX_CG_BREAKDOWN = 0, 1, 2, 3
 REASONS = {0: "converged", 1: "maximum_iterations", 2: "nonpositive_curvature", 3: "numerical_breakdown"}
+LMX_RNG_PCG32, LMX_RNG_XOSHIRO256PP, LMX_RNG_PHILOX4X32 = 0, 1, 2
+LMX_ELEM_F32, LMX_ELEM_F64, LMX_ELEM_C64, LMX_ELEM_C128 = 0, 1, 2, 3
+LMX_FILL_UNIFORM, LMX_FILL_GAUSSIAN = 0, 1
 
 
 class LmxError(RuntimeError):
@@ -44,6 +47,17 @@ class CGResultC(ctypes.Structure):
 
 class TuneResult(ctypes.Structure):
     _fields_ = [("rhs_block", ctypes.c_int32), ("block_size", ctypes.c_int32), ("ms_per_apply", ctypes.c_double)]
+
+
+class RNGKey(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("sweep", ctypes.c_uint64), ("direction", ctypes.c_uint32),
+                ("color", ctypes.c_uint32), ("subgroup", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class RNGDraws(ctypes.Structure):
+    _fields_ = [("words", ctypes.c_int32), ("uniform32", ctypes.c_int32), ("uniform64", ctypes.c_int32),
+                ("normal32", ctypes.c_int32), ("normal64", ctypes.c_int32), ("bounded", ctypes.c_int32),
+                ("upper", ctypes.c_uint32), ("reserved", ctypes.c_int32)]
 
 
 APPLY_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
@@ -87,6 +101,12 @@ SIGNATURES = {
     "lmx_ta_project": (I, [P, P, P, P]),
     "lmx_wilson_force": (I, [P, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
     "lmx_autotune": (I, [P, I, I, ctypes.c_char_p, ctypes.POINTER(TuneResult)]),
+    "lmx_site_rng_draw": (I, [P, ctypes.POINTER(RNGKey), I, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(RNGDraws),
+                              P, P, P, P, P, P]),
+    "lmx_randomize_padded": (I, [P, P, I, I, I, I, I, ctypes.POINTER(RNGKey), D, I, ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.POINTER(ctypes.c_int32)]),
+    "lmx_spinor_randomize": (I, [P, P, I, I, ctypes.POINTER(RNGKey), D, I, ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.POINTER(ctypes.c_int32)]),
 }
 
 _lib = None

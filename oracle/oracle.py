@@ -77,6 +77,14 @@ def lib():
         L.orc_clover_pullback.argtypes = [P, P, P, P, P, P, ctypes.c_double, ctypes.c_double]
         L.orc_dsfdu_clover.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, P, ctypes.POINTER(_OrcResult)]
+        U64, U32 = ctypes.c_uint64, ctypes.c_uint32
+        L.orc_splitmix64.argtypes, L.orc_splitmix64.restype = [U64], U64
+        L.orc_rng_stream_seed.argtypes, L.orc_rng_stream_seed.restype = [U64, U64, U32, U32, U32], U64
+        L.orc_pcg32_step.argtypes, L.orc_pcg32_step.restype = [ctypes.POINTER(U64), U64], U32
+        L.orc_philox4x32_10.argtypes = [P, P, P]
+        L.orc_site_rng_draw.argtypes = [U64, ctypes.c_int, U64, ctypes.c_int, P, U32, P, P, P, P, P, P]
+        L.orc_randomize_fill.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P,
+                                         ctypes.c_int, ctypes.c_double, U64, ctypes.c_int]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -274,3 +282,68 @@ def dsfdu(U, phi, L, kappa, phases=(1, 1, 1, -1), rtol=1e-12, maxiter=20000, ord
                     float(rtol), int(maxiter), int(order), _ptr(work), ctypes.byref(res))
     return G, eta, chi, CGResult(bool(res.converged), int(res.iterations), float(res.residual_norm),
                                  float(res.relative_residual), REASONS[int(res.reason)])
+
+
+# ---------------------------------------------------------------------------
+# Site random-number streams and lattice fills (lmx_rng_oracle.c)
+# ---------------------------------------------------------------------------
+RNG_ALGORITHMS = {"pcg32": 0, "xoshiro256++": 1, "philox4x32": 2}
+_ELEM = {np.dtype(np.float32): 0, np.dtype(np.float64): 1, np.dtype(np.complex64): 2, np.dtype(np.complex128): 3}
+
+
+def splitmix64(x: int) -> int:
+    return int(lib().orc_splitmix64(x))
+
+
+def rng_stream_seed(seed, sweep=0, direction=0, color=0, subgroup=0) -> int:
+    """_rng_stream_seed(RNGStreamKey(seed, sweep, direction, color, subgroup))."""
+    return int(lib().orc_rng_stream_seed(seed, sweep, direction, color, subgroup))
+
+
+def pcg32_step(state: int, inc: int):
+    st = ctypes.c_uint64(state)
+    out = lib().orc_pcg32_step(ctypes.byref(st), inc)
+    return int(st.value), int(out)
+
+
+def philox4x32_10(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_ptr(c), _ptr(k), _ptr(o))
+    return tuple(int(v) for v in o)
+
+
+def site_rng_draw(key, algorithm, first_site, nsites, words=0, uniform32=0, uniform64=0, normal32=0, normal64=0,
+                  bounded=0, upper=17):
+    """The per-site draw sequence of test/site_rng_gpu.jl:7-42; each output is
+    [nsites, count] (the reference's [count, nsites] column-major)."""
+    counts = np.array([words, uniform32, uniform64, normal32, normal64, bounded], dtype=np.int32)
+    out = {"words": np.zeros((nsites, words), np.uint32), "uniform32": np.zeros((nsites, uniform32), np.float32),
+           "uniform64": np.zeros((nsites, uniform64), np.float64), "normal32": np.zeros((nsites, normal32), np.float32),
+           "normal64": np.zeros((nsites, normal64), np.float64), "bounded": np.zeros((nsites, bounded), np.uint32)}
+    lib().orc_site_rng_draw(rng_stream_seed(*key), RNG_ALGORITHMS[algorithm], first_site, nsites, _ptr(counts),
+                            upper, *[_ptr(out[k]) for k in ("words", "uniform32", "uniform64", "normal32", "normal64",
+                                                            "bounded")])
+    return out
+
+
+def randomize_fill(dtype, nc1, nc2, L, key, gaussian=False, sigma=1.0, algorithm="philox4x32", offset=None,
+                   gdims=None):
+    """randomize_matrix! / randomize_gaussian_matrix! on the core of a field of
+    local extents L (any dimension) placed at offset in a lattice of extents
+    gdims (default: L is the whole lattice).  Returns [V, nc2, nc1] (site
+    x-fastest), i.e. the reference's [nc1, nc2, L...] column-major core."""
+    dt = np.dtype(dtype)
+    D = len(L)
+    Lc = (ctypes.c_int * D)(*[int(v) for v in L])
+    off = (ctypes.c_int * D)(*([0] * D if offset is None else [int(v) for v in offset]))
+    gd = (ctypes.c_int * D)(*[int(v) for v in (L if gdims is None else gdims)])
+    out = np.zeros((volume_any(L), nc2, nc1), dtype=dt)
+    lib().orc_randomize_fill(_ptr(out), _ELEM[dt], nc1, nc2, D, Lc, off, gd, int(bool(gaussian)), float(sigma),
+                             rng_stream_seed(*key), RNG_ALGORITHMS[algorithm])
+    return out
+
+
+def volume_any(L):
+    return int(np.prod([int(v) for v in L]))
