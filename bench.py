@@ -1,16 +1,18 @@
 """Benchmark of the Wilson D^dagger D CG path on one MI355X.
 
 Workload (N=1): 32^4 Wilson D^dagger D, weak-field SU(3) links
-U = exp(i 0.2 H) (seeded), Gaussian sources, kappa 0.124, antiperiodic in t,
+U = exp(i 0.2 H) (seeded), kappa 0.124, antiperiodic in t, the reference
+benchmark's Gaussian sources (source r: randomize_gaussian_matrix! with
+sigma = sqrt(1/2), seed 1235 + r, Philox4x32, generated on the device),
 FP64 true relative residual 1e-10, zero start.
   - plain FP64 GPU CG (the reference algorithm cg!, one RHS), and
   - the mixed-precision multi-RHS CG (FP16 links, inner vectors as 16-bit
     fixed point with a per-site scale, FP32 math; FP64 reliable updates /
     reductions / solution) on `--nrhs` sources.
 One step = one full solve of the mixed multi-RHS batch; `value` is its
-time-to-solution per RHS.  Inputs are generated on the host (numpy, seeded)
-and uploaded before the timed region; the timed region holds device work
-only (plus the solver's host convergence polls).
+time-to-solution per RHS.  Links are generated on the host (numpy, seeded)
+and uploaded, sources on the device, all before the timed region; the timed
+region holds device work only (plus the solver's host convergence polls).
 
 Also measured: the FP64 D_W launch (roofline kernel, 960 B/site algorithmic:
 links 576 + psi 192 + out 192) timed with HIP events on the library stream,
@@ -149,11 +151,12 @@ def main():
     n = args.nrhs
     log(f"bench: generating inputs {L} weak-field eps={args.eps}, {n} Gaussian sources")
     U = gen.links(L, "weak", 20240601, eps=args.eps)
-    b = gen.gaussian_spinor(L, 20240602, nrhs=n)
     lat = lmx.Lattice(L, (1, 1, 1, -1))
     Ug = lat.gauge_from_numpy(U)
     D = lmx.WilsonDiracOperator4D(Ug, args.kappa)
-    bg = lat.spinor_from_numpy(b)
+    # benchmark/ddagd_cg/ddagd_cg.jl:773-774 with --seed = 1234 + r for source r
+    bg = lat.spinor(n)
+    lmx.randomize_gaussian_matrix_(bg, [lmx.RNGStreamKey(1235 + r) for r in range(n)], sigma=0.5 ** 0.5)
     b0 = bg.rhs(0)
     x1, r, p, Ap, tmp = (lat.spinor() for _ in range(5))
     N = lmx.DdagDOp(D, tmp)
@@ -227,7 +230,7 @@ def main():
     traffic, traffic_src = measured_traffic(L)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(U, b[0], L, args.kappa, args.cpu_iters, res64.iterations)
+        cpu = cpu_baseline(U, bg.rhs(0).to_numpy(), L, args.kappa, args.cpu_iters, res64.iterations)
         log(f"bench: cpu baseline {cpu['sample']}")
 
     out = {
@@ -244,7 +247,8 @@ def main():
         "dtype": {"h16": "16-bit fixed-point inner spinors, f16 links, f32 math, f64 reliable updates",
                   "f16": "f16 links, f32 inner spinors and math, f64 reliable updates",
                   "f32": "f32 inner links, spinors and math, f64 reliable updates"}[args.precision],
-        "data": "synthetic (seeded weak-field SU(3) links eps=0.2, Gaussian sources)",
+        "data": "synthetic (seeded weak-field SU(3) links eps=0.2; the reference benchmark's Gaussian sources, "
+                "Philox seed 1235 + r, sigma sqrt(1/2), generated on the device)",
         "config": {"workload": f"Wilson D^dagger D CG {'x'.join(map(str, L))}, kappa {args.kappa}, rtol "
                                f"{args.rtol}, {n} RHS mixed vs 1 RHS FP64",
                    "lattice": list(L), "kappa": args.kappa, "rhs": n, "precision": args.precision},
