@@ -359,6 +359,19 @@ def dot(A: SpinorField, B: SpinorField) -> complex:
     return complex(out[0], out[1])
 
 
+def add_matrix_(C: SpinorField, A: SpinorField, alpha: complex = 1.0) -> SpinorField:
+    """C += alpha * A (add_matrix!, src/LinearAlgebras/linearalgebra_D.jl:2882-2911:
+    u += alpha * v per entry, unfused).  Real alpha is axpby(alpha, A, 1, C)
+    (1 * C is exact and the sum commutes); complex alpha forms the product with
+    the library's complex scale first, then adds."""
+    z = complex(alpha)
+    if z.imag == 0.0:
+        return axpby_(z.real, A, 1.0, C)
+    t = A.similar()
+    check(lib().lmx_scale(A.lattice.ctx, t.ptr, A.ptr, z.real, z.imag, A.nrhs))
+    return axpby_(1.0, t, 1.0, C)
+
+
 def clear_matrix_(A: SpinorField) -> SpinorField:
     A.data.zero_()
     return A

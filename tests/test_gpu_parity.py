@@ -105,6 +105,17 @@ def test_blas_matches_oracle(gpu):
     assert d == ref
     lmx.axpby_(0.37, A, -1.25, B)
     assert np.array_equal(B.to_numpy(), 0.37 * a + (-1.25) * b)
+    # add_matrix!(C, A, alpha): u += alpha * v per entry, every product and sum
+    # rounded separately (numpy's elementwise float64 operations)
+    B = lat.spinor_from_numpy(b)
+    lmx.add_matrix_(B, A, 0.75)
+    assert np.array_equal(B.to_numpy(), b + 0.75 * a)
+    B = lat.spinor_from_numpy(b)
+    lmx.add_matrix_(B, A, 0.3 - 1.1j)
+    ar, ai = 0.3, -1.1
+    pr = ar * a.real - ai * a.imag
+    pi = ar * a.imag + ai * a.real
+    assert np.array_equal(B.to_numpy(), (b.real + pr) + 1j * (b.imag + pi))
 
 
 def test_clover_matches_oracle(gpu):
