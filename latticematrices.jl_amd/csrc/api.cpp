@@ -543,14 +543,14 @@ int lmx_wilson_force(lmx_wilson* op, void* G, void* F, const void* phi, void* et
     const size_t fb = 12 * (size_t)c->V;
     double2* w = (double2*)work;
     double2 *r = w, *p = w + fb, *Ap = w + 2 * fb, *tmp = w + 3 * fb;
-    LMX_HIP(hipMemsetAsync(eta, 0, fb * sizeof(double2), c->stream));
+    LMX_TRY(launch_zero(c, (double2*)eta, (int64_t)fb));
     LMX_TRY(lmx_cg_ddagd(op, eta, phi, r, p, Ap, tmp, params, result));
     Epilogue e{0, nullptr, nullptr, nullptr, nullptr};
     LMX_TRY(launch_wilson_f64(c, op->links, op->clover64, op->kappa, op->csw, (double2*)chi, (const double2*)eta, 1, 0,
                               e));
     // dresult = -2 chi (the dot pullback of g = -2 Re dot(chi, D eta))
     LMX_TRY(launch_scale(c, tmp, (const double2*)chi, -2.0, 0.0, (int64_t)fb));
-    LMX_HIP(hipMemsetAsync(G, 0, 36 * (size_t)c->V * sizeof(double2), c->stream));
+    LMX_TRY(launch_zero(c, (double2*)G, 36 * (int64_t)c->V));
     LMX_TRY(operator_pullback(op, (double2*)G, tmp, (const double2*)eta, -op->kappa));
     if (F) LMX_TRY(launch_ta_project(c, (double2*)F, op->links, (const double2*)G));
     LMX_HIP(hipStreamSynchronize(c->stream));

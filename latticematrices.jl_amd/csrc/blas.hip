@@ -259,6 +259,21 @@ int launch_axpby(lmx_context* ctx, double a, const double2* X, double b, double2
     return LMX_OK;
 }
 
+// Zero fill as a kernel rather than hipMemsetAsync: the entry points that clear
+// buffers may be captured into a caller's HIP graph, where a kernel node
+// replays like every other launch.
+__global__ void __launch_bounds__(256) k_zero(double2* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = make_double2(0.0, 0.0);
+}
+
+int launch_zero(lmx_context* ctx, double2* p, int64_t n) {
+    if (n == 0) return LMX_OK;
+    hipLaunchKernelGGL(k_zero, grid_for(n), dim3(256), 0, ctx->stream, p, n);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
 int launch_scale(lmx_context* ctx, double2* out, const double2* in, double sr, double si, int64_t n) {
     if (n == 0) return LMX_OK;
     hipLaunchKernelGGL(k_scale, grid_for(n), dim3(256), 0, ctx->stream, out, in, sr, si, n);

@@ -372,7 +372,7 @@ int launch_clover_pullback(lmx_context* ctx, double2* dU, const double2* U, doub
                            const double2* psi, double coefficient) {
     if (ctx->V == 0) return LMX_OK;
     static const GammaProducts gp = gamma_products();
-    LMX_HIP(hipMemsetAsync(dF, 0, 6 * 9 * (size_t)ctx->V * sizeof(double2), ctx->stream));
+    LMX_TRY(launch_zero(ctx, dF, 6 * 9 * (int64_t)ctx->V));
     hipLaunchKernelGGL(k_clover_cotangent, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, dF, dres, psi,
                        (int)ctx->V, coefficient, gp);
     hipLaunchKernelGGL(k_clover_links_pullback, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, dU, U,

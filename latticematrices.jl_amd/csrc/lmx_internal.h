@@ -116,7 +116,14 @@ int launch_wilson_padded(lmx_context* ctx, const double2* U, const double2* clov
 // partials (partials + r * pstride): 2 = |out|^2 on D; 3 = on D^dagger,
 // out = Ap is not stored but r -= alpha Ap, x += alpha p (p = aux; r in the
 // spinor format, x FP32) with |r|^2, skipping RHS whose done flag is set;
-// the x update can be deferred to the next iteration (apply_x, pending).
+// the x update can be deferred over a group of iterations (apply_x, pmask).
+//
+// The mixed solver's x_inner update is batched over groups of kXBatch
+// iterations: phase j < kXBatch - 1 of a group defers alpha_j p_j (p_j stays
+// in ring slot j, bit j of the RHS's mask marks it), phase kXBatch - 1 adds
+// the marked terms and its own alpha p -- one x_inner read and write per
+// group instead of one per iteration.
+constexpr int kXBatch = 4;
 struct LoArgs {
     int link_prec, spinor_prec;
     const void* links;
@@ -137,16 +144,16 @@ struct LoArgs {
     float* r_norm;
     void* x;  // FP32 component pairs, float4 [nrhs][6][V]
     const double* alpha;
-    const double* alpha_prev;
+    const double* alpha_q;  // deferred alphas: slot j of RHS r at alpha_q[r * astride + j]
     int astride;
     const int* done;
-    const int* pending;
+    const int* pmask;       // deferred slots of RHS r (bit j) at pmask[r * dstride]
     int dstride;
-    // x update batched over two iterations: apply_x == 0 defers alpha p;
-    // apply_x == 1 adds alpha_prev pprev (where pending) and alpha p
+    // apply_x == 0 defers alpha p; apply_x == 1 adds alpha_q[j] pq[j] for the
+    // marked slots j (oldest first) and then alpha p
     int apply_x;
-    const void* pprev;
-    const float* pprev_norm;
+    const void* pq[kXBatch - 1];
+    const float* pq_norm[kXBatch - 1];
 };
 int launch_wilson_lo(lmx_context* ctx, const LoArgs& a);
 
@@ -160,6 +167,7 @@ enum { FIN_OUT = 0, FIN_BB = 1, FIN_RR0 = 2, FIN_RRNEW = 3, FIN_PAP = 4 };
 int launch_finalize(lmx_context* ctx, int nb, int kind, bool want_im, CGState* st, double* out2);
 int launch_dot(lmx_context* ctx, const double2* A, const double2* B, int fin_kind, CGState* st, double* out2);
 int launch_axpby(lmx_context* ctx, double a, const double2* X, double b, double2* Y, int64_t n);
+int launch_zero(lmx_context* ctx, double2* p, int64_t n);
 int launch_scale(lmx_context* ctx, double2* out, const double2* in, double sr, double si, int64_t n);
 int launch_cg_setup_r(lmx_context* ctx, const double2* b, const double2* Ap, double2* r, double2* p, CGState* st);
 int launch_cg_update_xr(lmx_context* ctx, double2* x, double2* r, const double2* p, const double2* Ap, CGState* st);

@@ -29,9 +29,9 @@ namespace lmx {
 
 struct MixRhs {
     double bnorm, tol, rr, rr_prev, pAp, alpha, beta, rref, rtrue;
-    double alpha_prev;  // alpha of the deferred x_inner += alpha p
+    double alpha_q[kXBatch - 1];  // alphas of the deferred x_inner += alpha p, by ring slot
     int done, converged, reason, since, iterations, updates;
-    int pending;        // a deferred alpha_prev p_prev awaits the next iteration (or a flush)
+    int pmask;  // ring slots whose alpha p awaits the group's last iteration (or a flush)
     int pad;
 };
 struct MixBatch {
@@ -51,7 +51,7 @@ struct InnerVec {
 
 struct MixPtrs {
     float4* xlo;  // inner solution: FP32 component pairs, float4 [nrhs][6][V]
-    InnerVec r, p[2], t;  // p double-buffered: iteration k reads p[k & 1] and writes p[(k + 1) & 1]
+    InnerVec r, p[kXBatch], t;  // p ring: iteration k reads p[k % kXBatch], writes p[(k + 1) % kXBatch]
     double2* r64;
     double2* t64;
     double* partials;
@@ -85,8 +85,7 @@ static size_t layout(lmx_context* ctx, int nrhs, int sp, MixPtrs* P, char* base)
     MixPtrs q;
     q.xlo = (float4*)take(f32);
     q.r = inner();
-    q.p[0] = inner();
-    q.p[1] = inner();
+    for (int j = 0; j < kXBatch; ++j) q.p[j] = inner();
     q.t = inner();
     q.r64 = (double2*)take(f64);
     q.t64 = (double2*)take(f64);
@@ -123,7 +122,8 @@ __device__ __forceinline__ void retire(MixRhs* st, MixBatch* batch, int reason, 
 enum { MX_INIT = 0, MX_PAP = 1, MX_RR = 2, MX_TRUE = 3 };
 
 // flag: MX_TRUE -- run even without a reliable-update request (final);
-//       MX_RR   -- this iteration deferred its x_inner += alpha p.
+//       MX_RR   -- the iteration's phase j in its group: j < kXBatch - 1
+//                  deferred its x_inner += alpha p into ring slot j.
 template <int KIND>
 __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ partials, int nb, MixRhs* rhs,
                                                 MixBatch* batch, double rtol, double atol, int flag) {
@@ -134,10 +134,14 @@ __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ parti
     if (st->done) return;
     if (threadIdx.x == 0) {
         if (KIND == MX_RR) {
-            st->pending = flag;
-            if (flag) st->alpha_prev = st->alpha;
+            if (flag < kXBatch - 1) {
+                st->alpha_q[flag] = st->alpha;
+                st->pmask |= 1 << flag;
+            } else {
+                st->pmask = 0;  // the group's last iteration applied them
+            }
         }
-        if (KIND == MX_TRUE) st->pending = 0;  // k_mx_accum folded the deferred term
+        if (KIND == MX_TRUE) st->pmask = 0;  // k_mx_accum folded the deferred terms
     }
     __shared__ double lds4[4];
     const double A = finish_partials(partials + (size_t)(2 * i) * nb, nb, lds4);
@@ -240,29 +244,47 @@ __global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta,
     }
 }
 
-// reliable update, part 1: x += x_lo (+ the deferred alpha p) in FP64, x_lo = 0
+// reliable update, part 1: x += x_lo (+ the deferred alpha p of the marked
+// ring slots, oldest first) in FP64, x_lo = 0
 template <class SI>
-__global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float4* __restrict__ xlo, SI pcur, int V,
-                                                  const MixRhs* rhs, const MixBatch* batch, int force) {
+struct RingSlots {
+    SI p[kXBatch - 1];
+};
+template <class SI>
+__global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float4* __restrict__ xlo, RingSlots<SI> ring,
+                                                  int V, const MixRhs* rhs, const MixBatch* batch, int force) {
     const int i = blockIdx.y;
     if ((!force && !batch->do_update) || rhs[i].done) return;
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= V) return;
     const size_t o = (size_t)i * 12 * V, o4 = (size_t)i * 6 * V;
-    cx<float> q[12];
-    const bool pend = rhs[i].pending != 0;  // this iteration deferred alpha p (p not yet updated)
-    const double ap = rhs[i].alpha_prev;
-    if (pend) pcur.ld12(i, s, q);
+    const int mask = rhs[i].pmask;  // slots not yet overwritten by a later p update
+    double d[24];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) d[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < kXBatch - 1; ++j) {
+        if (mask & (1 << j)) {
+            const double ap = rhs[i].alpha_q[j];
+            cx<float> q[12];
+            ring.p[j].ld12(i, s, q);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                d[2 * k] += ap * (double)q[k].re;
+                d[2 * k + 1] += ap * (double)q[k].im;
+            }
+        }
+    }
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
         const float4 l = xlo[o4 + (size_t)m * V + s];
         const size_t e0 = o + (size_t)(2 * m) * V + s, e1 = e0 + V;
         double a0 = (double)l.x, b0 = (double)l.y, a1 = (double)l.z, b1 = (double)l.w;
-        if (pend) {
-            a0 += ap * (double)q[2 * m].re;
-            b0 += ap * (double)q[2 * m].im;
-            a1 += ap * (double)q[2 * m + 1].re;
-            b1 += ap * (double)q[2 * m + 1].im;
+        if (mask) {
+            a0 += d[4 * m];
+            b0 += d[4 * m + 1];
+            a1 += d[4 * m + 2];
+            b1 += d[4 * m + 3];
         }
         const double2 h0 = x[e0], h1 = x[e1];
         x[e0] = make_double2(h0.x + a0, h0.y + b0);
@@ -360,11 +382,11 @@ static LoArgs lo_args(lmx_wilson* op, int link_prec, int spinor_prec) {
 //   t = D p, with the block partials of |t|^2 = p.Ap (FP64 sums), then
 //   (after the finalize has set alpha) Ap = D^dagger t, kept in registers:
 //   r -= alpha Ap with the block partials of |r|^2, and the x_inner update
-//   batched over two iterations: an even iteration defers alpha p (its p
-//   stays in its buffer, the finalize marks it pending), the odd iteration
-//   adds alpha_prev p_prev + alpha p -- the x_inner traffic of one iteration
-//   for two.  A reliable update after an even iteration folds the pending
-//   term into x in k_mx_accum.
+//   batched over groups of kXBatch iterations: phase j < kXBatch - 1 defers
+//   alpha p (its p stays in ring slot j, the finalize records alpha and
+//   marks the slot), the group's last phase adds the marked terms and its own
+//   alpha p -- the x_inner traffic of one iteration for kXBatch.  A reliable
+//   update inside a group folds the marked terms into x in k_mx_accum.
 static_assert(sizeof(MixRhs) % sizeof(double) == 0, "MixRhs is addressed in doubles and ints");
 
 static int apply_d_norm(MixRun& R, int e) {
@@ -393,19 +415,21 @@ static int apply_ddag_xr(MixRun& R, int e) {
     a.in_norm = R.w.t.n;
     a.aux = R.w.p[e].p;
     a.aux_norm = R.w.p[e].n;
-    a.pprev = R.w.p[1 - e].p;
-    a.pprev_norm = R.w.p[1 - e].n;
-    a.apply_x = e;
+    for (int j = 0; j < kXBatch - 1; ++j) {
+        a.pq[j] = R.w.p[j].p;
+        a.pq_norm[j] = R.w.p[j].n;
+    }
+    a.apply_x = e == kXBatch - 1 ? 1 : 0;
     a.flags = LMX_DAGGER;
     a.epi = 3;
     a.r = R.w.r.p;
     a.r_norm = R.w.r.n;
     a.x = R.w.xlo;
     a.alpha = (const double*)((const char*)R.w.rhs + offsetof(MixRhs, alpha));
-    a.alpha_prev = (const double*)((const char*)R.w.rhs + offsetof(MixRhs, alpha_prev));
+    a.alpha_q = (const double*)((const char*)R.w.rhs + offsetof(MixRhs, alpha_q));
     a.astride = (int)(sizeof(MixRhs) / sizeof(double));
     a.done = (const int*)((const char*)R.w.rhs + offsetof(MixRhs, done));
-    a.pending = (const int*)((const char*)R.w.rhs + offsetof(MixRhs, pending));
+    a.pmask = (const int*)((const char*)R.w.rhs + offsetof(MixRhs, pmask));
     a.dstride = (int)(sizeof(MixRhs) / sizeof(int));
     a.partials = R.w.partials;
     a.pstride = 2L * nblocks(R.ctx->V);
@@ -418,12 +442,12 @@ static void enqueue_true_res(MixRun& R, dim3 grid, int force) {
     hipLaunchKernelGGL((k_mx_true_res<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V), V,
                        R.w.partials, R.w.rhs, R.w.batch, force);
 }
-// p[1 - e] = r + beta p[e]
+// p[(e + 1) % kXBatch] = r + beta p[e]
 template <class SI, class SO>
 static void enqueue_update_p(MixRun& R, dim3 grid, int e) {
     const int V = (int)R.ctx->V;
     hipLaunchKernelGGL((k_mx_update_p<SI, SO>), grid, dim3(256), 0, R.ctx->stream, in_of<SI>(R.w.r, V),
-                       in_of<SI>(R.w.p[e], V), out_of<SO>(R.w.p[1 - e], V), V, R.w.rhs);
+                       in_of<SI>(R.w.p[e], V), out_of<SO>(R.w.p[(e + 1) % kXBatch], V), V, R.w.rhs);
 }
 template <class SO>
 static void enqueue_init(MixRun& R, dim3 grid) {
@@ -431,18 +455,24 @@ static void enqueue_init(MixRun& R, dim3 grid) {
     hipLaunchKernelGGL((k_mx_init<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V),
                        out_of<SO>(R.w.p[0], V), R.w.xlo, V, R.w.partials);
 }
-// reliable-update part 1 (gated, or forced at the end): x += x_inner (+ pending alpha p[e])
-static void enqueue_accum(MixRun& R, dim3 grid, int e, int force) {
+// reliable-update part 1 (gated, or forced at the end): x += x_inner (+ the marked alpha p)
+template <class SI>
+static RingSlots<SI> ring_of(MixRun& R) {
+    RingSlots<SI> g;
+    for (int j = 0; j < kXBatch - 1; ++j) g.p[j] = in_of<SI>(R.w.p[j], (int)R.ctx->V);
+    return g;
+}
+static void enqueue_accum(MixRun& R, dim3 grid, int force) {
     const int V = (int)R.ctx->V;
     if (R.spinor_prec == LMX_H16)
-        hipLaunchKernelGGL((k_mx_accum<SpH16In>), grid, dim3(256), 0, R.ctx->stream, R.x, R.w.xlo,
-                           in_of<SpH16In>(R.w.p[e], V), V, R.w.rhs, R.w.batch, force);
+        hipLaunchKernelGGL((k_mx_accum<SpH16In>), grid, dim3(256), 0, R.ctx->stream, R.x, R.w.xlo, ring_of<SpH16In>(R),
+                           V, R.w.rhs, R.w.batch, force);
     else
-        hipLaunchKernelGGL((k_mx_accum<SpF32In>), grid, dim3(256), 0, R.ctx->stream, R.x, R.w.xlo,
-                           in_of<SpF32In>(R.w.p[e], V), V, R.w.rhs, R.w.batch, force);
+        hipLaunchKernelGGL((k_mx_accum<SpF32In>), grid, dim3(256), 0, R.ctx->stream, R.x, R.w.xlo, ring_of<SpF32In>(R),
+                           V, R.w.rhs, R.w.batch, force);
 }
 
-// One iteration with parity e (iteration k has e = k & 1).
+// One iteration with phase e (iteration k has e = k % kXBatch).
 static int enqueue_iteration(MixRun& R, int e) {
     lmx_context* c = R.ctx;
     const bool h16 = R.spinor_prec == LMX_H16;
@@ -453,9 +483,8 @@ static int enqueue_iteration(MixRun& R, int e) {
     LMX_TRY(apply_d_norm(R, e));
     hipLaunchKernelGGL(k_mx_fin<MX_PAP>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
     LMX_TRY(apply_ddag_xr(R, e));
-    hipLaunchKernelGGL(k_mx_fin<MX_RR>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0,
-                       e == 0 ? 1 : 0);
-    enqueue_accum(R, grid, e, 0);
+    hipLaunchKernelGGL(k_mx_fin<MX_RR>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, e);
+    enqueue_accum(R, grid, 0);
     LMX_TRY(apply_ddagd64(R, R.w.r64, R.x, R.w.t64, &bt->do_update));
     if (h16)
         enqueue_true_res<SpH16Out>(R, grid, 0);
@@ -536,10 +565,10 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
                        R.w.batch, params->rtol, params->atol, 0);
     LMX_HIP(hipGetLastError());
 
-    // iterations between host polls; even with a graph, so every replay
-    // starts at an even iteration (the x_inner batching pairs iterations)
+    // iterations between host polls; a multiple of kXBatch with a graph, so
+    // every replay starts a group (the x_inner batching groups iterations)
     int chunk = params->check_every > 0 ? params->check_every : 10;
-    if (params->use_graph && (chunk & 1)) ++chunk;
+    if (params->use_graph) chunk = (chunk + kXBatch - 1) / kXBatch * kXBatch;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     if (params->use_graph) {
@@ -550,7 +579,7 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
         int rc = LMX_OK;
         if (e == hipSuccess) {
-            for (int k = 0; k < chunk && rc == LMX_OK; ++k) rc = enqueue_iteration(R, k & 1);
+            for (int k = 0; k < chunk && rc == LMX_OK; ++k) rc = enqueue_iteration(R, k % kXBatch);
             e = hipStreamEndCapture(cap, &graph);
         }
         c->stream = saved;
@@ -576,7 +605,7 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
             launched += chunk;
         } else {
             const int n = std::min(chunk, params->maxiter - launched);
-            for (int k = 0; k < n && rc == LMX_OK; ++k) rc = enqueue_iteration(R, (launched + k) & 1);
+            for (int k = 0; k < n && rc == LMX_OK; ++k) rc = enqueue_iteration(R, (launched + k) % kXBatch);
             if (rc != LMX_OK) break;
             launched += n;
         }
@@ -585,8 +614,8 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     if (graph) (void)hipGraphDestroy(graph);
     LMX_TRY(rc);
     // RHS stopped by maxiter: fold the inner solution in and report the true residual
-    // the last iteration's p buffer holds a still pending alpha p
-    enqueue_accum(R, grid, launched > 0 ? ((launched - 1) & 1) : 0, 1);
+    // the ring slots marked in pmask hold the still deferred alpha p
+    enqueue_accum(R, grid, 1);
     LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
     if (R.spinor_prec == LMX_H16)
         enqueue_true_res<SpH16Out>(R, grid, 1);
@@ -716,12 +745,12 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     lx.r_norm = norm_of(rbuf);
     lx.x = xbuf;
     lx.alpha = alpha;
-    lx.alpha_prev = alpha;
+    lx.alpha_q = alpha;
     lx.astride = 1;
     lx.done = done;
-    lx.pending = done;  // zeros: no deferred term
+    lx.pmask = done;  // zeros: no deferred term
     lx.dstride = 1;
-    lx.apply_x = 1;     // the x-updating (odd) iteration
+    lx.apply_x = 1;   // the x-updating iteration (last of its group)
     lx.partials = parts;
     lx.pstride = 2L * (long)nbk;
     auto pair = [&]() {

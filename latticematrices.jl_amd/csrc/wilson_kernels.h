@@ -189,21 +189,22 @@ enum { EPI_NONE = 0, EPI_PAP = 1, EPI_NORM = 2, EPI_XR = 3 };
 
 struct NoEx {};
 // Extra fields of EPI_XR, all offset to the launch's first RHS.  The x
-// update is batched over two iterations: with apply_x == 0 the kernel leaves
-// x alone (alpha p is deferred, p kept in its buffer); with apply_x == 1 it
-// adds the deferred alpha_prev p_prev (where pending) and alpha p.
+// update is batched over groups of kXBatch iterations: with apply_x == 0 the
+// kernel leaves x alone (alpha p is deferred, p kept in its ring slot); with
+// apply_x == 1 it adds the deferred alpha_q[j] pq[j] of the marked slots,
+// oldest first, and then alpha p.
 template <class In, class Out>
 struct XrEx {
     In rin;        // r (read) ...
     Out rout;      // ... and written in place
     SpX4In xin;    // inner solution x (FP32, component pairs)
     SpX4Out xout;
-    In pprev;      // the previous iteration's p (apply_x == 1)
+    In pq[kXBatch - 1];        // ring slots of the deferred p (apply_x == 1)
     const double* alpha;       // alpha of RHS r at alpha[r * astride]
-    const double* alpha_prev;  // deferred alpha at alpha_prev[r * astride]
+    const double* alpha_q;     // deferred alpha of slot j at alpha_q[r * astride + j]
     int astride;
     const int* done;           // retired RHS (no update) at done[r * dstride]
-    const int* pending;        // a deferred alpha p is pending at pending[r * dstride]
+    const int* pmask;          // deferred slots (bit j) at pmask[r * dstride]
     int dstride;
     int apply_x;
 };
@@ -248,13 +249,17 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
             cx<R> pv[12], xv[12];
             aux.ld12(r, s, pv);
             ex.xin.ld12(r, s, xv);
-            if (ex.pending[(long)r * ex.dstride]) {
-                const R ap = R(ex.alpha_prev[(long)r * ex.astride]);
-                cx<R> qv[12];
-                ex.pprev.ld12(r, s, qv);
+            const int m = ex.pmask[(long)r * ex.dstride];
 #pragma unroll
-                for (int k = 0; k < 12; ++k)
-                    xv[k] = mk<R>(fmaR(ap, qv[k].re, xv[k].re), fmaR(ap, qv[k].im, xv[k].im));
+            for (int j = 0; j < kXBatch - 1; ++j) {
+                if (m & (1 << j)) {
+                    const R ap = R(ex.alpha_q[(long)r * ex.astride + j]);
+                    cx<R> qv[12];
+                    ex.pq[j].ld12(r, s, qv);
+#pragma unroll
+                    for (int k = 0; k < 12; ++k)
+                        xv[k] = mk<R>(fmaR(ap, qv[k].re, xv[k].re), fmaR(ap, qv[k].im, xv[k].im));
+                }
             }
 #pragma unroll
             for (int k = 0; k < 12; ++k) xv[k] = mk<R>(fmaR(a, pv[k].re, xv[k].re), fmaR(a, pv[k].im, xv[k].im));
@@ -925,14 +930,16 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
                              Out::at(a.r, a.r_norm, V, r0),
                              SpX4In::at(a.x, nullptr, V, r0),
                              SpX4Out::at(a.x, nullptr, V, r0),
-                             In::at(a.pprev ? a.pprev : a.aux, a.pprev ? a.pprev_norm : a.aux_norm, V, r0),
+                             {},
                              a.alpha + (long)r0 * a.astride,
-                             a.alpha_prev + (long)r0 * a.astride,
+                             a.alpha_q + (long)r0 * a.astride,
                              a.astride,
                              a.done + (long)r0 * a.dstride,
-                             a.pending + (long)r0 * a.dstride,
+                             a.pmask + (long)r0 * a.dstride,
                              a.dstride,
                              a.apply_x};
+            for (int j = 0; j < kXBatch - 1; ++j)
+                ex.pq[j] = In::at(a.pq[j] ? a.pq[j] : a.aux, a.pq[j] ? a.pq_norm[j] : a.aux_norm, V, r0);
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, XrEx<In, Out>, NR, EPI_XR, false, false>(
                 ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl)));
         } else if (a.epi == EPI_NORM) {

@@ -162,10 +162,12 @@ def test_many_rhs_split_over_launches_h16(gpu):
     assert all(r.converged for r in r1)
 
 
-@pytest.mark.parametrize("check_every", [1, 3, 7])
+@pytest.mark.parametrize("check_every", [1, 3, 6, 7])
 def test_batched_x_update_any_poll_interval(gpu, check_every):
-    # the x update pairs iterations (even ones defer alpha p); odd poll
-    # intervals, graph (rounded up to even) or eager, must not change the iterates
+    # the x update groups iterations by four (the first three defer alpha p
+    # into their ring slots, the fourth applies them); poll intervals that
+    # split groups, graph (rounded up to a whole group) or eager, must not
+    # change the iterates, and reliable updates land inside groups
     L, kappa = (8, 4, 4, 8), 0.12
     U, b, lat, Ug, D = setup(L, "weak", 380, kappa, nrhs=2)
     bg = lat.spinor_from_numpy(b)
@@ -177,13 +179,14 @@ def test_batched_x_update_any_poll_interval(gpu, check_every):
         xs.append((x.to_numpy(), [r.iterations for r in res]))
     assert xs[0][1] == xs[1][1]
     assert np.array_equal(xs[0][0], xs[1][0])
-    # stopped by maxiter after an even (deferring) iteration: the pending term is folded in
-    x = lat.spinor(2)
-    res = lmx.cg_batched_(x, D, bg, precision="h16", maxiter=5, check_every=1)
-    assert res[0].reason == "maximum_iterations"
+    # stopped by maxiter with one, two or three deferred terms: all are folded in
     A = orc.NormalOp(orc.OP_DDAGD_WILSON, L, U=U, kappa=kappa)
-    tr = np.linalg.norm(A.apply(x.to_numpy()[0]) - b[0]) / np.linalg.norm(b[0])
-    assert abs(tr - res[0].true_relative_residual) <= 1e-6 * tr
+    for maxiter in (5, 6, 7):
+        x = lat.spinor(2)
+        res = lmx.cg_batched_(x, D, bg, precision="h16", maxiter=maxiter, check_every=1)
+        assert res[0].reason == "maximum_iterations"
+        tr = np.linalg.norm(A.apply(x.to_numpy()[0]) - b[0]) / np.linalg.norm(b[0])
+        assert abs(tr - res[0].true_relative_residual) <= 1e-6 * tr
 
 
 def test_rhs_count_limits(gpu):
