@@ -14,7 +14,9 @@
  *   spinor  F64: double2 [nrhs][12][V]    component k = spin*3 + colour
  *           F32: float2  [nrhs][12][V]
  *   links   F64: double2 [4][9][V]        element k = row*3 + col
- *           F32: float2  [4][9][V]    F16: half2 [4][9][V]
+ *           F32: float2  [4][9][V]
+ *           F16: per direction two 16-byte words (elements 0-3, 4-7) and one
+ *                4-byte word (element 8): int4 [4][2][V] then half2 [4][V]
  *   clover  F64: double2 [27][V]          6 anti-Hermitian planes (12,13,14,23,24,34):
  *           slots 0..8 hold the 18 imaginary diagonals (plane*3+i) in pairs,
  *           slots 9..26 the upper off-diagonals (plane*3 + {01,02,12}).
