@@ -141,6 +141,16 @@ int lmx_spinor_import(lmx_context* ctx, void* dst, const void* src_padded, int n
 /* Internal F64 SoA -> reference padded AoS; the halo is filled with the
  * phase-wrapped core values exactly as set_halo! would (:707-723). */
 int lmx_spinor_export(lmx_context* ctx, void* dst_padded, int nw, const void* src, int nrhs);
+/* LatticeMatrix(A, dim, PEs; nw, phases) on one rank
+ * (src/LatticeMatrices_core.jl:405-461) for any field: the core array
+ * A[nc1, nc2, L1, ..., L4] (column-major, element type elem = LMX_ELEM_*, no
+ * halo) -> the padded storage dst[nc1, nc2, L1+2nw, ..., L4+2nw] with the halo
+ * filled as set_halo! does (the context's boundary phases: real for real
+ * fields, +-1 for single precision). */
+int lmx_field_pad(lmx_context* ctx, void* dst_padded, const void* src_core, int elem, int nc1, int nc2, int nw);
+/* gather_matrix(ls) on one rank (src/LatticeMatrices_core.jl:936-993): the
+ * interior of padded storage as the core array A[nc1, nc2, L1, ..., L4]. */
+int lmx_field_gather(lmx_context* ctx, void* dst_core, const void* src_padded, int elem, int nc1, int nc2, int nw);
 /* Four reference link arrays A[3,3,L+2nw...] (complex128) -> internal F64 links. */
 int lmx_links_import(lmx_context* ctx, void* dst, const void* const src_padded[4], int nw);
 /* Internal F64 links -> four padded link arrays (halo periodic). */

@@ -280,6 +280,16 @@ int lmx_spinor_randomize(lmx_context* c, void* dst, int nrhs, int dist, const lm
     LMX_TRY(check_ctx(c));
     return launch_spinor_randomize(c, dst, nrhs, dist, keys, sigma, algorithm, global_offset, global_dims);
 }
+int lmx_field_pad(lmx_context* c, void* dst, const void* src, int elem, int nc1, int nc2, int nw) {
+    clear_error();
+    LMX_TRY(check_ctx(c));
+    return launch_field_pad(c, dst, src, elem, nc1, nc2, nw);
+}
+int lmx_field_gather(lmx_context* c, void* dst, const void* src, int elem, int nc1, int nc2, int nw) {
+    clear_error();
+    LMX_TRY(check_ctx(c));
+    return launch_field_unpad(c, dst, src, elem, nc1, nc2, nw);
+}
 int lmx_links_import(lmx_context* c, void* dst, const void* const src[4], int nw) {
     clear_error();
     LMX_TRY(check_ctx(c));

@@ -185,4 +185,170 @@ int links_export(lmx_context* ctx, void* const dst[4], int nw, const void* src) 
     return LMX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Generic fields: any NC1 x NC2 matrix per site, Float32 / Float64 /
+// ComplexF32 / ComplexF64, element e = ic + NC1 * jc of a site contiguous.
+// ---------------------------------------------------------------------------
+int make_halo_phases(const lmx_context* ctx, int elem, HaloPhases* hp) {
+    const bool real_el = elem == LMX_ELEM_F32 || elem == LMX_ELEM_F64;
+    const bool single = elem == LMX_ELEM_F32 || elem == LMX_ELEM_C64;
+    for (int d = 0; d < 4; ++d) {
+        const double re = ctx->g.ph_re[d], im = ctx->g.ph_im[d];
+        hp->skip[d] = ctx->g.phase_one[d];
+        LMX_CHECK(!real_el || im == 0.0, LMX_ERR_ARG,
+                  "a real-valued field cannot take the complex boundary phase of dimension %d", d + 1);
+        LMX_CHECK(!single || hp->skip[d] || ctx->g.phase_neg[d], LMX_ERR_ARG,
+                  "single-precision fields with a halo support boundary phases +-1 only (dimension %d)", d + 1);
+        hp->plus_re[d] = re;
+        hp->plus_im[d] = im;
+        if (real_el) {
+            hp->minus_re[d] = 1.0 / re;  // inv(x::Float64)
+            hp->minus_im[d] = 0.0;
+        } else {
+            hp->minus_re[d] = ctx->g.iph_re[d];
+            hp->minus_im[d] = ctx->g.iph_im[d];
+        }
+    }
+    return LMX_OK;
+}
+
+template <class T>
+struct C2 {
+    T re, im;
+};
+
+// LatticeMatrix(A, dim, PEs) on one rank (src/LatticeMatrices_core.jl:405-461):
+// padded[ps][e] = core[wrapped site][e] times the phases of the directions the
+// ghost wraps across (exchange_dim_local!, :707-723).  One thread per element.
+template <class T, bool CPLX>
+__global__ void __launch_bounds__(256) k_field_pad(void* __restrict__ dst, const void* __restrict__ src, Geom g,
+                                                   Pad p, int ne, HaloPhases ph) {
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= p.Vp * ne) return;
+    const long ps = t / ne;
+    const int e = (int)(t - ps * ne);
+    long rem = ps;
+    int x[4], w[4];
+    for (int d = 0; d < 4; ++d) {
+        int c = (int)(rem % p.P[d]) - p.nw;
+        rem /= p.P[d];
+        w[d] = 0;
+        if (c < 0) {
+            c += g.L[d];
+            w[d] = -1;
+        } else if (c >= g.L[d]) {
+            c -= g.L[d];
+            w[d] = 1;
+        }
+        x[d] = c;
+    }
+    const long s = x[0] + (long)g.stride[1] * x[1] + (long)g.stride[2] * x[2] + (long)g.stride[3] * x[3];
+    if constexpr (CPLX) {
+        C2<T> v = ((const C2<T>*)src)[s * ne + e];
+        for (int d = 0; d < 4; ++d) {
+            if (w[d] == 0 || ph.skip[d]) continue;
+            const T pr = (T)(w[d] > 0 ? ph.plus_re[d] : ph.minus_re[d]);
+            const T pi = (T)(w[d] > 0 ? ph.plus_im[d] : ph.minus_im[d]);
+            const T nr = v.re * pr - v.im * pi;
+            const T ni = v.re * pi + v.im * pr;
+            v.re = nr;
+            v.im = ni;
+        }
+        ((C2<T>*)dst)[t] = v;
+    } else {
+        T v = ((const T*)src)[s * ne + e];
+        for (int d = 0; d < 4; ++d) {
+            if (w[d] == 0 || ph.skip[d]) continue;
+            v = v * (T)(w[d] > 0 ? ph.plus_re[d] : ph.minus_re[d]);
+        }
+        ((T*)dst)[t] = v;
+    }
+}
+
+// gather_matrix on one rank (:936-993): the interior of the padded array, as
+// raw elements of W bytes.  One thread per element.
+template <class W>
+__global__ void __launch_bounds__(256) k_field_unpad(W* __restrict__ dst, const W* __restrict__ src, Geom g, Pad p,
+                                                     int ne) {
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long)g.V * ne) return;
+    const long s = t / ne;
+    const int e = (int)(t - s * ne);
+    int x[4];
+    site_coords((int)s, g, x);
+    dst[t] = src[padded_site(x, p) * ne + e];
+}
+
+static int check_field(const lmx_context* ctx, int elem, int nc1, int nc2, int nw) {
+    LMX_CHECK(elem >= LMX_ELEM_F32 && elem <= LMX_ELEM_C128, LMX_ERR_ARG,
+              "fields support Float32, Float64, ComplexF32 and ComplexF64 elements (got code %d)", elem);
+    LMX_CHECK(nc1 >= 1 && nc2 >= 1 && nc1 * nc2 <= 4096, LMX_ERR_ARG, "bad matrix shape %d x %d", nc1, nc2);
+    return check_nw(ctx, nw);
+}
+
+int launch_field_pad(lmx_context* ctx, void* dst, const void* src, int elem, int nc1, int nc2, int nw) {
+    LMX_CHECK(dst && src, LMX_ERR_ARG, "NULL field");
+    LMX_TRY(check_field(ctx, elem, nc1, nc2, nw));
+    HaloPhases ph;
+    if (nw > 0) {
+        LMX_TRY(make_halo_phases(ctx, elem, &ph));
+    } else {
+        for (int d = 0; d < 4; ++d) {
+            ph.skip[d] = 1;
+            ph.plus_re[d] = ph.minus_re[d] = 1.0;
+            ph.plus_im[d] = ph.minus_im[d] = 0.0;
+        }
+    }
+    const Pad p = make_pad(ctx, nw);
+    const int ne = nc1 * nc2;
+    const long n = p.Vp * ne;
+    if (n == 0) return LMX_OK;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    switch (elem) {
+        case LMX_ELEM_F32:
+            hipLaunchKernelGGL((k_field_pad<float, false>), grid, dim3(256), 0, ctx->stream, dst, src, ctx->g, p, ne, ph);
+            break;
+        case LMX_ELEM_F64:
+            hipLaunchKernelGGL((k_field_pad<double, false>), grid, dim3(256), 0, ctx->stream, dst, src, ctx->g, p, ne,
+                               ph);
+            break;
+        case LMX_ELEM_C64:
+            hipLaunchKernelGGL((k_field_pad<float, true>), grid, dim3(256), 0, ctx->stream, dst, src, ctx->g, p, ne, ph);
+            break;
+        default:
+            hipLaunchKernelGGL((k_field_pad<double, true>), grid, dim3(256), 0, ctx->stream, dst, src, ctx->g, p, ne,
+                               ph);
+            break;
+    }
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+int launch_field_unpad(lmx_context* ctx, void* dst, const void* src, int elem, int nc1, int nc2, int nw) {
+    LMX_CHECK(dst && src, LMX_ERR_ARG, "NULL field");
+    LMX_TRY(check_field(ctx, elem, nc1, nc2, nw));
+    const Pad p = make_pad(ctx, nw);
+    const int ne = nc1 * nc2;
+    const long n = ctx->V * (long)ne;
+    if (n == 0) return LMX_OK;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    switch (elem) {
+        case LMX_ELEM_F32:
+            hipLaunchKernelGGL((k_field_unpad<unsigned>), grid, dim3(256), 0, ctx->stream, (unsigned*)dst,
+                               (const unsigned*)src, ctx->g, p, ne);
+            break;
+        case LMX_ELEM_F64:
+        case LMX_ELEM_C64:
+            hipLaunchKernelGGL((k_field_unpad<unsigned long long>), grid, dim3(256), 0, ctx->stream,
+                               (unsigned long long*)dst, (const unsigned long long*)src, ctx->g, p, ne);
+            break;
+        default:
+            hipLaunchKernelGGL((k_field_unpad<uint4>), grid, dim3(256), 0, ctx->stream, (uint4*)dst, (const uint4*)src,
+                               ctx->g, p, ne);
+            break;
+    }
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
 }  // namespace lmx

@@ -178,6 +178,22 @@ int launch_cg_init_state(lmx_context* ctx, CGState* st, double rtol, double atol
 int launch_pullback_f64(lmx_context* ctx, double2* dU, const double2* dres, const double2* psi, double coefficient);
 int launch_ta_project(lmx_context* ctx, double2* F, const double2* U, const double2* G);
 // clover part of the Wilson-clover link pullback (dF: 6*9*V scratch)
+// Boundary phases of a halo refill for fields of element type elem
+// (LMX_ELEM_*), exchange_dim_local! (src/LatticeMatrices_core.jl:707-723):
+// a plus ghost is the wrapped core value times the phase, a minus ghost times
+// its inverse, direction by direction, skipped where the phase is exactly 1.
+// Real fields take real phases (inverse 1/x, as inv(x::Float64)), complex
+// ones the context's complex inverse; single-precision fields take +-1 only,
+// so that the phase and its inverse are exact in any precision.
+struct HaloPhases {
+    int skip[4];
+    double plus_re[4], plus_im[4], minus_re[4], minus_im[4];
+};
+int make_halo_phases(const lmx_context* ctx, int elem, HaloPhases* hp);
+// Generic padded-field construction and interior gather (any NC1 x NC2, any
+// element type): LatticeMatrix(A, dim, PEs) and gather_matrix on one rank.
+int launch_field_pad(lmx_context* ctx, void* dst, const void* src, int elem, int nc1, int nc2, int nw);
+int launch_field_unpad(lmx_context* ctx, void* dst, const void* src, int elem, int nc1, int nc2, int nw);
 int launch_site_rng_draw(lmx_context* ctx, const lmx_rng_key* key, int algorithm, uint64_t first_site, int nsites,
                          const lmx_rng_draws* n, uint32_t* words, float* u32, double* u64, float* n32, double* n64,
                          uint32_t* bnd);

@@ -282,8 +282,7 @@ struct FillGeom {
     long Vp;      // threads: padded volume (halo) or core volume
     uint64_t goff[4];
     uint64_t gstride[4];
-    int skip[4];  // phase exactly one in direction d
-    double plus_re[4], plus_im[4], minus_re[4], minus_im[4];
+    HaloPhases ph;
 };
 
 template <class T>
@@ -331,10 +330,10 @@ __global__ void __launch_bounds__(256) k_rng_fill_padded(void* __restrict__ out,
     const T sig = (T)sigma;
     site_fill<T, CPLX, GAUSS>(s, ne, sig, [&](int e, T re, T im) {
         for (int d = 0; d < 4; ++d) {
-            if (w[d] == 0 || g.skip[d]) continue;
-            const T pr = (T)(w[d] > 0 ? g.plus_re[d] : g.minus_re[d]);
+            if (w[d] == 0 || g.ph.skip[d]) continue;
+            const T pr = (T)(w[d] > 0 ? g.ph.plus_re[d] : g.ph.minus_re[d]);
             if constexpr (CPLX) {
-                const T pi = (T)(w[d] > 0 ? g.plus_im[d] : g.minus_im[d]);
+                const T pi = (T)(w[d] > 0 ? g.ph.plus_im[d] : g.ph.minus_im[d]);
                 const T nr = re * pr - im * pi;
                 const T ni = re * pi + im * pr;
                 re = nr;
@@ -479,33 +478,12 @@ int launch_randomize_padded(lmx_context* ctx, void* C, int elem, int nc1, int nc
     FillGeom g;
     LMX_TRY(make_fill_geom(ctx, nw, goff, gdims, g));
     if (g.halo) {
-        // Real fields take real phases (phases::SVector{D,T} of the field's
-        // element type); single-precision fields take +-1 so that the phase
-        // and its inverse are exact in any precision.
-        for (int d = 0; d < 4; ++d) {
-            const double re = ctx->g.ph_re[d], im = ctx->g.ph_im[d];
-            g.skip[d] = ctx->g.phase_one[d];
-            const bool real_el = elem == LMX_ELEM_F32 || elem == LMX_ELEM_F64;
-            LMX_CHECK(!real_el || im == 0.0, LMX_ERR_ARG,
-                      "a real-valued field cannot take the complex boundary phase of dimension %d", d + 1);
-            const bool single = elem == LMX_ELEM_F32 || elem == LMX_ELEM_C64;
-            LMX_CHECK(!single || g.skip[d] || ctx->g.phase_neg[d], LMX_ERR_ARG,
-                      "single-precision fields with a halo support boundary phases +-1 only (dimension %d)", d + 1);
-            g.plus_re[d] = re;
-            g.plus_im[d] = im;
-            if (real_el) {
-                g.minus_re[d] = 1.0 / re;  // inv(x::Float64)
-                g.minus_im[d] = 0.0;
-            } else {
-                g.minus_re[d] = ctx->g.iph_re[d];
-                g.minus_im[d] = ctx->g.iph_im[d];
-            }
-        }
+        LMX_TRY(make_halo_phases(ctx, elem, &g.ph));
     } else {
         for (int d = 0; d < 4; ++d) {
-            g.skip[d] = 1;
-            g.plus_re[d] = g.minus_re[d] = 1.0;
-            g.plus_im[d] = g.minus_im[d] = 0.0;
+            g.ph.skip[d] = 1;
+            g.ph.plus_re[d] = g.ph.minus_re[d] = 1.0;
+            g.ph.plus_im[d] = g.ph.minus_im[d] = 0.0;
         }
     }
     if (g.Vp == 0) return LMX_OK;

@@ -101,6 +101,8 @@ SIGNATURES = {
     "lmx_ta_project": (I, [P, P, P, P]),
     "lmx_wilson_force": (I, [P, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
     "lmx_autotune": (I, [P, I, I, ctypes.c_char_p, ctypes.POINTER(TuneResult)]),
+    "lmx_field_pad": (I, [P, P, P, I, I, I, I]),
+    "lmx_field_gather": (I, [P, P, P, I, I, I, I]),
     "lmx_site_rng_draw": (I, [P, ctypes.POINTER(RNGKey), I, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(RNGDraws),
                               P, P, P, P, P, P]),
     "lmx_randomize_padded": (I, [P, P, I, I, I, I, I, ctypes.POINTER(RNGKey), D, I, ctypes.POINTER(ctypes.c_int32),

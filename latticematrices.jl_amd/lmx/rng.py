@@ -17,6 +17,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ArgumentError, check, lib
+from .fields import ELEM
 from .lattice import SpinorField
 
 try:
@@ -87,10 +88,7 @@ def site_rng_draw(lattice, key: RNGStreamKey, nsites: int, *, algorithm: str = "
     return res
 
 
-_ELEM = {}
-if torch is not None:
-    _ELEM = {torch.float32: _lib.LMX_ELEM_F32, torch.float64: _lib.LMX_ELEM_F64,
-             torch.complex64: _lib.LMX_ELEM_C64, torch.complex128: _lib.LMX_ELEM_C128}
+_ELEM = ELEM
 
 
 def _placement(global_offset, global_dims):

@@ -133,3 +133,27 @@ def free_field_eigen(L, n, kappa, phases=(1, 1, 1, -1)):
     for mu in range(4):
         Dt = Dt - kappa * (2 * np.cos(p[mu]) * np.eye(4) - 2j * GAMMA[mu] * np.sin(p[mu]))
     return ph, Dt
+
+
+def halo_fill(core, dims, nw, phases):
+    """set_halo! on one rank (src/LatticeMatrices_core.jl:707-723) of a core
+    array [L4, L3, L2, L1, NC2, NC1] (C order): periodic copy into a halo of
+    width nw, plus ghosts times the phase, minus ghosts times its inverse,
+    direction by direction (a corner takes the phases of every direction it
+    wraps across)."""
+    a = np.asarray(core).reshape(dims[3], dims[2], dims[1], dims[0], *np.asarray(core).shape[-2:])
+    if nw == 0:
+        return a.copy()
+    a = np.pad(a, [(nw, nw)] * 4 + [(0, 0), (0, 0)], mode="wrap")
+    for d in range(4):
+        ph = phases[d]
+        if ph == 1:
+            continue
+        ax = 3 - d
+        lo = [slice(None)] * 6
+        hi = [slice(None)] * 6
+        lo[ax] = slice(0, nw)
+        hi[ax] = slice(dims[d] + nw, dims[d] + 2 * nw)
+        a[tuple(lo)] = a[tuple(lo)] * a.dtype.type(1 / ph)
+        a[tuple(hi)] = a[tuple(hi)] * a.dtype.type(ph)
+    return a

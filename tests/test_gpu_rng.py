@@ -13,6 +13,7 @@ import torch
 
 import oracle as orc
 import lmx
+from dense_reference import halo_fill
 
 pytestmark = pytest.mark.gpu
 
@@ -45,26 +46,7 @@ def test_site_draws_match_oracle(gpu, alg):
 
 
 def _expected_padded(core, dims, nw, phases):
-    """set_halo! on one rank (src/LatticeMatrices_core.jl:707-723): periodic
-    copy, plus ghosts times the phase, minus ghosts times its inverse,
-    direction by direction."""
-    nc2, nc1 = core.shape[-2:]
-    a = core.reshape(dims[3], dims[2], dims[1], dims[0], nc2, nc1)
-    if nw == 0:
-        return a
-    a = np.pad(a, [(nw, nw)] * 4 + [(0, 0), (0, 0)], mode="wrap")
-    for d in range(4):
-        ph = phases[d]
-        if ph == 1:
-            continue
-        ax = 3 - d
-        idx_m = [slice(None)] * 6
-        idx_p = [slice(None)] * 6
-        idx_m[ax] = slice(0, nw)
-        idx_p[ax] = slice(dims[d] + nw, dims[d] + 2 * nw)
-        a[tuple(idx_m)] = a[tuple(idx_m)] * a.dtype.type(1 / ph)
-        a[tuple(idx_p)] = a[tuple(idx_p)] * a.dtype.type(ph)
-    return a
+    return halo_fill(core, dims, nw, phases)
 
 
 @pytest.mark.parametrize("alg", ALGS)
