@@ -1,0 +1,57 @@
+"""The pseudofermion action as a differentiable function with a hand-written
+reverse rule.
+
+S(U, phi) = Re phi^dagger (D(U)^dagger D(U))^-1 phi for the Wilson or
+Wilson-clover operator, as the reference differentiates it with its Enzyme
+rules (dSFdU, ext/LatticeMatricesEnzymeExt.jl:387-460; the Wilson link
+pullback, ext/AD/WilsonDiracOperator.jl:93-170; the clover pullback,
+src/Operators/WilsonCloverPullback.jl:342-389).  The reverse rule does not
+record the CG: the forward pass solves for eta = (D^dagger D)^-1 phi once
+and, when the links need a gradient, applies the hand-written link pullback
+to (eta, chi = D eta) on the device (lmx_wilson_force).  Gradients are with
+respect to the real storage of the tensors, (re, im) pairs:
+
+    dS/dU   = G = dS/dRe U + i dS/dIm U  (the Enzyme convention; its (re, im)
+              pairs are exactly the gradient of the links tensor)
+    dS/dphi = 2 eta                      (S is Hermitian-quadratic in phi)
+"""
+from __future__ import annotations
+
+import torch
+
+from .lattice import GaugeField, SpinorField, WilsonDiracOperator4D, dSFdU, dot
+
+
+class PseudofermionAction(torch.autograd.Function):
+    """S = PseudofermionAction.apply(links, phi, lattice, kappa, csw, rtol, maxiter)
+
+    links: GaugeField.data (float64 [4, 9, V, 2]); phi: SpinorField.data of
+    one field (float64 [1, 12, V, 2]).  Returns S as a 0-dim float64 tensor
+    on the device.  Convergence failures of the solve raise RuntimeError."""
+
+    @staticmethod
+    def forward(ctx, links, phi, lattice, kappa, csw=0.0, rtol=1e-12, maxiter=20000):
+        U = GaugeField(lattice, links.detach())
+        D = WilsonDiracOperator4D(U, float(kappa), float(csw))
+        p = SpinorField(lattice, 1, phi.detach())
+        G = GaugeField(lattice)
+        res, eta, _chi = dSFdU(G, D, p, rtol=rtol, maxiter=maxiter)
+        if not res.converged:
+            raise RuntimeError(f"pseudofermion action: CG did not converge ({res.reason}, {res.iterations} iterations)")
+        S = dot(p, eta).real
+        ctx.save_for_backward(G.data, eta.data)
+        return torch.tensor(S, dtype=torch.float64, device=links.device)
+
+    @staticmethod
+    def backward(ctx, gS):
+        G, eta = ctx.saved_tensors
+        gl = gS * G if ctx.needs_input_grad[0] else None
+        gp = 2.0 * gS * eta if ctx.needs_input_grad[1] else None
+        return gl, gp, None, None, None, None, None
+
+
+def pseudofermion_action_ad(U: GaugeField, phi: SpinorField, kappa: float, csw: float = 0.0, rtol: float = 1e-12,
+                            maxiter: int = 20000):
+    """S(U, phi) with autograd through the hand-written adjoint; U.data and
+    phi.data may require grad."""
+    return PseudofermionAction.apply(U.data, phi.data, U.lattice, kappa, csw, rtol, maxiter)
