@@ -153,7 +153,24 @@ def test_clover_48x96_mixed_4rhs(gpu):
     x = lat.spinor(4)
     tune = lmx.autotune(D, "h16", 4)
     blk = tune["rhs_block"]
-    print(f"  autotuned shape TL={blk >> 12} per={(blk >> 4) & 0xff} nr={blk & 15}", flush=True)
+    print(f"  autotuned shape TL={blk >> 12} per={(blk >> 4) & 0xff} nr={blk & 15}: fused D / D^dagger pair "
+          f"{tune['ms_per_apply'] * 1e3:.0f} us per application per RHS", flush=True)
+    # the FP64 clover D_W against the HBM roofline: links 576 + psi 192 +
+    # clover cache 432 + out 192 = 1392 algorithmic bytes per site
+    y = lat.spinor()
+    b0 = bg.rhs(0)
+    lmx.mul_(y, D, b0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        lmx.mul_(y, D, b0)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / 10 * 1e3
+    gbs = 1392 * lat.V / (us * 1e-6) / 1e9
+    print(f"  FP64 clover D_W: {us:.0f} us per application, {gbs:.0f} GB/s algorithmic = {gbs / 8000:.3f} of the "
+          f"8 TB/s HBM peak", flush=True)
+    del y
     lmx.cg_batched_(x, D, bg, precision="h16", rtol=1e-3)  # warm up
     lmx.clear_matrix_(x)
     torch.cuda.synchronize()
