@@ -27,6 +27,12 @@ int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover,
             ctx, o, i, l, F, aux, none, kappa, ccoef, 1, 1, flags, clover != nullptr, epi, 256)));
         return launch_finalize(ctx, nblocks(ctx->V), FIN_PAP, false, e.st, nullptr);
     }
+    // several Wilson RHS: two per work item share each link load (the same
+    // per-RHS arithmetic, so bit-identical to one per work item); the clover
+    // variant would spill at two, so it keeps one
+    if (nrhs >= 2 && clover == nullptr)
+        return dispatch<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, NoEx, 2, EPI_NONE, true, true>(
+            ctx, o, i, l, F, i, none, kappa, ccoef, (nrhs + 1) / 2, nrhs, flags, false, epi, 256);
     return dispatch<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, NoEx, 1, EPI_NONE, true, true>(
         ctx, o, i, l, F, i, none, kappa, ccoef, nrhs, nrhs, flags, clover != nullptr, epi, 256);
 }
