@@ -95,15 +95,16 @@ def measured_traffic(L):
 
 
 def lo_dslash_bytes(precision, n):
-    """Algorithmic bytes per site per RHS of one operator application of the
-    mixed solver's fused pair (the autotuner times D with |D p|^2 and
-    D^dagger with the x / r update, and reports the pair / 2): links once per
-    launch of n RHS in each; D reads p, writes t; D^dagger reads t, p, r,
-    x_inner (FP32) and writes r, x_inner."""
+    """Algorithmic bytes per site per RHS of one operator application,
+    averaged as the autotuner times them: a group of four inner iterations,
+    each D with |D p|^2 (reads p, writes t) and D^dagger with the r update
+    (reads t and r, writes r), the last D^dagger also adding the four alpha p
+    (reads four p) to x_inner (FP32, read and written); links once per launch
+    of n RHS.  Per application: links / n + 3 spinors + (4 spinors + 192) / 8."""
     links = 288.0 if precision == "f32" else 144.0
     spinor = 52.0 if precision == "h16" else 96.0
-    pair = 2 * links / n + 2 * spinor + 4 * spinor + 2 * 96.0
-    return pair / 2
+    group = 8 * links / n + 20 * spinor + 4 * spinor + 2 * 96.0
+    return group / 8
 
 
 def copy_calibration(lat):

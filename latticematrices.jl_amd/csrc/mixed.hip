@@ -641,8 +641,10 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
 }
 
 // ---------------------------------------------------------------------------
-// Autotuner: time one inner iteration's operator pair as the solver runs it
-// (D with |D p|^2, D^dagger with the fused x / r update) for every RHS
+// Autotuner: time one group of kXBatch inner iterations' operator pairs as
+// the solver runs them (D with |D p|^2, D^dagger with the fused r update,
+// the x update deferred on all but the last, which adds the deferred terms
+// from their own buffers) for every RHS
 // blocking (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,4})
 // and streaming depth (t-slices per workgroup: automatic, 4, 8) and keep
 // the fastest per (volume, precision, nrhs, arch); optionally persisted as
@@ -698,13 +700,15 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     }
     LMX_TRY(ensure_lo_links(op));
     const int sp = spinor_prec_of(precision);
-    // buffers of one inner iteration: p, t = D p, r, x_inner, alpha, done, partials
+    // buffers of one group of inner iterations: p, t = D p, r, x_inner,
+    // partials, alpha (one per RHS and deferred slot), done (zeros), the
+    // deferred-slot masks (all slots marked) and the deferred p slots
     const size_t fb = align256(lmx_spinor_bytes(c, sp, nrhs));
     const size_t xb = align256(12 * (size_t)c->V * nrhs * sizeof(float2));
     const size_t nbk = (size_t)nblocks(c->V);
     const size_t pb = align256(2 * nbk * nrhs * sizeof(double));
-    const size_t ab = align256(nrhs * sizeof(double)), db = align256(nrhs * sizeof(int));
-    const size_t total = 3 * fb + xb + pb + ab + db;
+    const size_t ab = align256((nrhs + kXBatch) * sizeof(double)), db = align256(nrhs * sizeof(int));
+    const size_t total = 3 * fb + xb + pb + ab + 2 * db + (kXBatch - 1) * fb;
     char* buf = nullptr;
     if (hipMalloc(&buf, total) != hipSuccess) {
         set_error("autotune buffer allocation of %zu bytes failed", total);
@@ -718,6 +722,10 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     double* parts = (double*)(xbuf + xb);
     const double* alpha = (const double*)((char*)parts + pb);
     const int* done = (const int*)((const char*)alpha + ab);
+    int* masks = (int*)((char*)done + db);
+    char* qbuf = (char*)masks + db;  // kXBatch - 1 deferred p slots
+    static_assert(kXBatch >= 2 && kXBatch <= 8, "deferred-slot masks are filled bytewise");
+    (void)hipMemsetAsync(masks, (1 << (kXBatch - 1)) - 1, db, c->stream);  // slot bits 0..kXBatch-2 set
     auto norm_of = [&](char* q) -> float* {  // per-site scales follow the 16-bit data
         return sp == LMX_H16 ? (float*)(q + (size_t)nrhs * c->V * 3 * sizeof(int4)) : nullptr;
     };
@@ -748,14 +756,22 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     lx.alpha_q = alpha;
     lx.astride = 1;
     lx.done = done;
-    lx.pmask = done;  // zeros: no deferred term
+    lx.pmask = masks;  // every deferred slot marked
     lx.dstride = 1;
-    lx.apply_x = 1;   // the x-updating iteration (last of its group)
+    for (int j = 0; j < kXBatch - 1; ++j) {
+        lx.pq[j] = qbuf + (size_t)j * fb;
+        lx.pq_norm[j] = norm_of(qbuf + (size_t)j * fb);
+    }
     lx.partials = parts;
     lx.pstride = 2L * (long)nbk;
-    auto pair = [&]() {
-        int rc2 = launch_wilson_lo(c, ld);
-        return rc2 == LMX_OK ? launch_wilson_lo(c, lx) : rc2;
+    auto group = [&]() {
+        int rc2 = LMX_OK;
+        for (int j = 0; j < kXBatch && rc2 == LMX_OK; ++j) {
+            lx.apply_x = j == kXBatch - 1 ? 1 : 0;
+            rc2 = launch_wilson_lo(c, ld);
+            if (rc2 == LMX_OK) rc2 = launch_wilson_lo(c, lx);
+        }
+        return rc2;
     };
     const int per_opts[5] = {1, 2, 4, 6, 12};
     int rc = LMX_OK;
@@ -774,15 +790,15 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
             if (nr == 3 || nr > per || per % nr != 0) continue;  // built: 1, 2, 4 per work item
             const int blk = (tl << 12) | (per << 4) | nr;
             ld.rhs_block = lx.rhs_block = blk;
-            for (int k = 0; k < 2 && rc == LMX_OK; ++k) rc = pair();
-            const int reps = 5;
+            rc = group();
+            const int reps = 2;
             (void)hipEventRecord(e0, c->stream);
-            for (int k = 0; k < reps && rc == LMX_OK; ++k) rc = pair();
+            for (int k = 0; k < reps && rc == LMX_OK; ++k) rc = group();
             (void)hipEventRecord(e1, c->stream);
             if (hipEventSynchronize(e1) != hipSuccess) rc = LMX_ERR_HIP;
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            const double us = 1e3 * ms / reps / nrhs / 2;  // per operator application and RHS
+            const double us = 1e3 * ms / reps / kXBatch / nrhs / 2;  // per operator application and RHS
             if (rc == LMX_OK && us < best_us) {
                 best_us = us;
                 best = blk;
