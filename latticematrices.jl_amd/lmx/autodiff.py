@@ -19,7 +19,8 @@ from __future__ import annotations
 
 import torch
 
-from .lattice import GaugeField, SpinorField, WilsonDiracOperator4D, dSFdU, dot
+from .lattice import (GaugeField, SpinorField, WilsonDiracOperator4D, _ensure_clover_cache_current, adjoint,
+                      dSFdU, dot, mul_, mul_cached_clover_, wilson_pullback_)
 
 
 class PseudofermionAction(torch.autograd.Function):
@@ -55,3 +56,56 @@ def pseudofermion_action_ad(U: GaugeField, phi: SpinorField, kappa: float, csw: 
     """S(U, phi) with autograd through the hand-written adjoint; U.data and
     phi.data may require grad."""
     return PseudofermionAction.apply(U.data, phi.data, U.lattice, kappa, csw, rtol, maxiter)
+
+
+class MulCachedClover(torch.autograd.Function):
+    """result = MulCachedClover.apply(links, psi, cache, U)
+
+    mul_cached_clover!(result, cache, U1..U4, psi) as a differentiable
+    function, with the reference's reverse rule
+    (ext/AD/WilsonDiracCloverOperator.jl:348-444): the link cotangent is the
+    analytic Wilson + clover pullback (wilson_clover_link_pullback!,
+    src/Operators/WilsonCloverPullback.jl:328-389, coefficient -kappa on the
+    device), the spinor cotangent D^dagger dresult.  links must be U.data
+    (float64 [4, 9, V, 2]), psi a SpinorField's data (float64 [n, 12, V, 2]);
+    the result is a new float64 [n, 12, V, 2] tensor.  Gradients are with
+    respect to the (re, im) storage, the Enzyme convention dL/dRe + i dL/dIm."""
+
+    @staticmethod
+    def forward(ctx, links, psi, cache, U):
+        if links.data_ptr() != U.data.data_ptr():
+            raise ValueError("links must be the GaugeField's data tensor")
+        L = cache.lattice
+        x = SpinorField(L, psi.shape[0], psi.detach())
+        out = SpinorField(L, psi.shape[0])
+        mul_cached_clover_(out, cache, U, x)
+        ctx.cache, ctx.U = cache, U
+        ctx.save_for_backward(links, psi)
+        return out.data
+
+    @staticmethod
+    def backward(ctx, g):
+        links, psi = ctx.saved_tensors  # raises if either was written in place since the forward pass
+        cache, U = ctx.cache, ctx.U
+        _ensure_clover_cache_current(cache, U)
+        L = cache.lattice
+        n = psi.shape[0]
+        dres = SpinorField(L, n, g.detach().to(torch.float64).contiguous())
+        gl = gp = None
+        if ctx.needs_input_grad[0]:
+            dU = GaugeField(L)
+            x = SpinorField(L, n, psi.detach())
+            for i in range(n):
+                wilson_pullback_(dU, cache, dres.rhs(i), x.rhs(i), -cache.kappa)
+            gl = dU.data
+        if ctx.needs_input_grad[1]:
+            dpsi = SpinorField(L, n)
+            mul_(dpsi, adjoint(cache), dres)
+            gp = dpsi.data
+        return gl, gp, None, None
+
+
+def mul_cached_clover_ad(cache: WilsonDiracOperator4D, U: GaugeField, psi: SpinorField):
+    """D(U) psi through mul_cached_clover_ with autograd; U.data and psi.data
+    may require grad.  Returns the result tensor [n, 12, V, 2]."""
+    return MulCachedClover.apply(U.data, psi.data, cache, U)

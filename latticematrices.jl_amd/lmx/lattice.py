@@ -156,10 +156,26 @@ class GaugeField:
         if data is None:
             data = torch.zeros((4, 9, lattice.V, 2), dtype=torch.float64, device=lattice.device)
         self.data = data
+        self._core_epoch = 0
 
     @property
     def ptr(self):
         return ctypes.c_void_p(_ptr(self.data))
+
+    def mark_halo_dirty_(self):
+        """mark_halo_dirty!(U) (src/LatticeMatrices_core.jl:196-203): record a
+        write to the links that torch's in-place operations do not count (one
+        through a raw device pointer, or through `tensor.data`), so that
+        mul_cached_clover_ refreshes the operator's derived copies."""
+        self._core_epoch += 1
+
+    @property
+    def core_epoch(self):
+        """The links' version: (explicit epoch, torch's in-place version
+        counter of .data, device address of .data).  Any in-place torch write,
+        a replaced .data tensor or mark_halo_dirty_ changes it (the reference's
+        halo_epoch.core, src/LatticeMatrices_core.jl:196-214)."""
+        return (self._core_epoch, self.data._version, self.data.data_ptr())
 
     @classmethod
     def from_numpy(cls, lattice: Lattice, U, nw: int = 0) -> "GaugeField":
@@ -206,6 +222,11 @@ class WilsonDiracOperator4D:
         check(lib().lmx_wilson_create(self.lattice.ctx, ctypes.byref(h), U.ptr, float(kappa), float(csw)))
         self._op = h
         self._kappa = float(kappa)
+        # refreshes of the derived copies (clover cache, low-precision links),
+        # and the links and their version those copies were built from
+        # (CloverCacheState, src/Operators/WilsonDiracCloverOperator.jl:417-432)
+        self.cache_epoch = 0
+        self._source = (U, U.core_epoch)
 
     @property
     def kappa(self):
@@ -219,8 +240,12 @@ class WilsonDiracOperator4D:
     def update_links(self, U: GaugeField | None = None):
         """update_clover!(operator[, U]) (src/Operators/WilsonDiracCloverOperator.jl:468-479)."""
         if U is not None:
+            if U.lattice.dims != self.lattice.dims:
+                raise ArgumentError(_lib.LMX_ERR_ARG, "links of another lattice geometry")
             self.U = U
         check(lib().lmx_wilson_update_links(self._op, self.U.ptr))
+        self.cache_epoch += 1
+        self._source = (self.U, self.U.core_epoch)
 
     def clover_planes(self):
         """The six clover field-strength planes [6, V, 3, 3] (CloverFieldStrength4D)."""
@@ -243,6 +268,41 @@ def WilsonDiracCloverOperator4D(U: GaugeField, kappa: float, cSW: float) -> Wils
     """D = D_W - kappa cSW sum_{mu<nu} gamma_mu gamma_nu F_mu,nu
     (src/Operators/WilsonDiracCloverOperator.jl:403-456)."""
     return WilsonDiracOperator4D(U, kappa, cSW)
+
+
+def _ensure_clover_cache_current(cache: WilsonDiracOperator4D, U: GaugeField) -> WilsonDiracOperator4D:
+    """_ensure_clover_cache_current! (src/Operators/WilsonDiracCloverOperator.jl:603-623):
+    refresh (update_clover!(cache, U)) unless the cache was built from this
+    very GaugeField at its current version."""
+    if not isinstance(cache, WilsonDiracOperator4D):
+        raise TypeError(f"not a Wilson(-clover) operator: {type(cache).__name__}")
+    if not isinstance(U, GaugeField):
+        raise TypeError(f"links must be a GaugeField, not {type(U).__name__}")
+    src, epoch = cache._source
+    if src is U and epoch == U.core_epoch:
+        return cache
+    cache.update_links(U)
+    return cache
+
+
+def mul_cached_clover_(result: SpinorField, cache: WilsonDiracOperator4D, U: GaugeField,
+                       psi: SpinorField) -> SpinorField:
+    """mul_cached_clover!(result, cache, U1, U2, U3, U4, psi)
+    (src/Operators/WilsonDiracCloverOperator.jl:625-651): apply the operator
+    with the links taken explicitly from U (U1..U4 are its four directions),
+    refreshing the clover cache first when U is not the GaugeField the cache
+    was built from or has been written since."""
+    _ensure_clover_cache_current(cache, U)
+    return mul_(result, cache, psi)
+
+
+def mul_cached_clover_adjoint_(result: SpinorField, cache: WilsonDiracOperator4D, U: GaugeField,
+                               psi: SpinorField) -> SpinorField:
+    """mul_cached_clover_adjoint!(result, cache, U1, U2, U3, U4, psi)
+    (src/Operators/WilsonDiracCloverOperator.jl:653-668): the adjoint, with the
+    same refreshed cache."""
+    _ensure_clover_cache_current(cache, U)
+    return mul_(result, adjoint(cache), psi)
 
 
 class WilsonDiracOperator4D_Donly:
