@@ -6,7 +6,7 @@ reference's LatticeMatrix operator API.
 """
 from . import _lib, generators
 from ._lib import ArgumentError, LmxError
-from .lattice import (Adjoint, add_matrix_, CGResult, DdagDOp, GaugeField, Lattice, ScaledOperator, SpinorField,
+from .lattice import (Adjoint, add_matrix_, cg, CGResult, DdagDOp, DiracOp, FieldPool, GaugeField, Lattice, ScaledOperator, SpinorField,
                       WilsonDiracCloverOperator4D, WilsonDiracOperator4D, WilsonDiracOperator4D_Donly, adjoint,
                       axpby_, cg_, clear_matrix_, dot, dSFdU, mul_, mul_padded_, pseudofermion_action, solve_, substitute_,
                       ta_project_, wilson_pullback_, cg_batched_, autotune, mul_cached_clover_,
@@ -17,7 +17,7 @@ from .rng import (DEFAULT_RANDOMIZE_SEED, RNGStreamKey, randomize_gaussian_matri
                   site_rng_draw)
 
 __all__ = [
-    "ArgumentError", "LmxError", "Adjoint", "CGResult", "DdagDOp", "GaugeField", "Lattice", "ScaledOperator",
+    "ArgumentError", "LmxError", "Adjoint", "cg", "CGResult", "DdagDOp", "DiracOp", "FieldPool", "GaugeField", "Lattice", "ScaledOperator",
     "SpinorField", "WilsonDiracCloverOperator4D", "WilsonDiracOperator4D", "WilsonDiracOperator4D_Donly",
     "adjoint", "add_matrix_", "axpby_", "cg_", "clear_matrix_", "dot", "dSFdU", "mul_", "mul_padded_", "pseudofermion_action", "solve_",
     "substitute_", "ta_project_", "wilson_pullback_", "generators", "cg_batched_", "autotune",

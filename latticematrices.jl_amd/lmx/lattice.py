@@ -328,12 +328,79 @@ def adjoint(A):
     return Adjoint(A)
 
 
+class FieldPool:
+    """PreallocatedArray with get_block / unused! (the reference's work-field
+    pool): fields made by `make` on first use, handed out and returned by
+    index; the pool grows when every field is in use.  A pool over a fixed
+    list (FieldPool.of) cannot grow."""
+
+    def __init__(self, make, num: int = 0):
+        self._make = make
+        self._fields = [None] * int(num)
+        self._used = [False] * int(num)
+
+    @classmethod
+    def of(cls, fields):
+        pool = cls(None)
+        pool._fields = list(fields)
+        pool._used = [False] * len(pool._fields)
+        return pool
+
+    def __len__(self):
+        return len(self._fields)
+
+    def get_block(self, n: int | None = None):
+        """One field and its index (n None), or lists of n fields and indices."""
+        k = 1 if n is None else int(n)
+        idx = [i for i, u in enumerate(self._used) if not u][:k]
+        while len(idx) < k:
+            if self._make is None:
+                raise ArgumentError(_lib.LMX_ERR_ARG, f"work-field pool of {len(self._fields)} fields has fewer "
+                                                      f"than {k} free")
+            self._fields.append(None)
+            self._used.append(False)
+            idx.append(len(self._fields) - 1)
+        for i in idx:
+            if self._fields[i] is None:
+                self._fields[i] = self._make()
+            self._used[i] = True
+        blk = [self._fields[i] for i in idx]
+        return (blk[0], idx[0]) if n is None else (blk, idx)
+
+    def unused_(self, idx):
+        for i in ([idx] if isinstance(idx, int) else idx):
+            self._used[i] = False
+
+
+class DiracOp:
+    """DiracOp(U, apply, apply_dag, p, phi; numtemp=4, numphitemp=4)
+    (src/Operators/DiracOperators.jl:21-46,48-77): an operator given by two
+    callbacks, apply(y, U, x, p, phitemps, temps) for D and apply_dag(...) for
+    D^dagger, each called by mul_ with four borrowed spinor work fields
+    (similar to phi) and four borrowed link work fields (GaugeField; U and the
+    link temporaries hold all four directions U1..U4 of the reference's
+    call).  adjoint(DiracOp) applies apply_dag; DdagDOp(D) without a temp
+    borrows one from D.phitemps."""
+
+    def __init__(self, U: GaugeField, apply, apply_dag, p, phi: SpinorField, numtemp: int = 4,
+                 numphitemp: int = 4):
+        self.U, self.apply, self.apply_dag, self.p = U, apply, apply_dag, p
+        self.temps = FieldPool(lambda: GaugeField(U.lattice), numtemp)
+        self.phitemps = FieldPool(phi.similar, numphitemp)
+
+
 @dataclass
 class DdagDOp:
     """Allocation-free normal operator D^dagger D with a caller-owned temp
-    (src/Operators/DiracOperators.jl:93-121)."""
+    (src/Operators/DiracOperators.jl:93-121).  DdagDOp(D) with a DiracOp and
+    no temp borrows the temporary from D.phitemps on each application (the
+    compatibility constructor, :98)."""
     D: object
-    temp: SpinorField
+    temp: SpinorField | None = None
+
+    def __post_init__(self):
+        if self.temp is None and not isinstance(self.D, DiracOp):
+            raise TypeError("DdagDOp(D) without a temporary field needs a DiracOp")
 
 
 def _wilson_handle(A):
@@ -355,6 +422,25 @@ def mul_(C: SpinorField, A, psi: SpinorField) -> SpinorField:
     if isinstance(A, Number):
         a = complex(A)
         check(lib().lmx_scale(C.lattice.ctx, C.ptr, psi.ptr, a.real, a.imag, C.nrhs))
+        return C
+    if isinstance(A, DiracOp) or (isinstance(A, Adjoint) and isinstance(A.parent, DiracOp)):
+        D = A if isinstance(A, DiracOp) else A.parent
+        fn = D.apply if D is A else D.apply_dag
+        temps, it = D.temps.get_block(4)
+        phis, ip = D.phitemps.get_block(4)
+        try:
+            fn(C, D.U, psi, D.p, phis, temps)
+        finally:
+            D.temps.unused_(it)
+            D.phitemps.unused_(ip)
+        return C
+    if isinstance(A, DdagDOp) and A.temp is None:
+        temp, i = A.D.phitemps.get_block()
+        try:
+            mul_(temp, A.D, psi)
+            mul_(C, adjoint(A.D), temp)
+        finally:
+            A.D.phitemps.unused_(i)
         return C
     if isinstance(A, DdagDOp):
         if A.temp is psi or A.temp is C:
@@ -478,7 +564,7 @@ def cg_(x, A, b, r, p, Ap, rtol=1e-10, atol=0.0, maxiter=5000, check_every=0) ->
     if int(maxiter) < 0:
         raise ArgumentError(_lib.LMX_ERR_ARG, "maxiter must be nonnegative")
     res = _lib.CGResultC()
-    if isinstance(A, DdagDOp) and isinstance(A.D, WilsonDiracOperator4D):
+    if isinstance(A, DdagDOp) and isinstance(A.D, WilsonDiracOperator4D) and A.temp is not None:
         check(lib().lmx_cg_ddagd(A.D._op, x.ptr, b.ptr, r.ptr, p.ptr, Ap.ptr, A.temp.ptr, ctypes.byref(params),
                                  ctypes.byref(res)))
         return CGResult.from_c(res)
@@ -506,14 +592,52 @@ def _wrap_ptr(ptr, lattice, cache):
     return cache[ptr]
 
 
-def solve_(x, A: DdagDOp, b, r, p, Ap, **kw) -> CGResult:
-    """solve!(x, A::DdagDOp, b, r, p, Ap; kwargs...) (DiracOperators.jl:130-132)."""
+def cg(x, A, b, temps, eps=1e-10, maxsteps=5000, verboselevel=2):
+    """cg(x, A, b, temps; eps, maxsteps, verboselevel)
+    (src/LinearAlgebras/cg.jl:140-198), the compatibility interface: r, p and
+    Ap are borrowed from temps (a FieldPool such as DiracOp.phitemps, or a
+    list of spinor fields), x holds the initial guess; returns None on
+    convergence and raises RuntimeError otherwise."""
+    pool = temps if isinstance(temps, FieldPool) else FieldPool.of(temps)
+    (r, p, Ap), idx = pool.get_block(3)
+    try:
+        res = cg_(x, A, b, r, p, Ap, rtol=eps, atol=0.0, maxiter=maxsteps)
+    finally:
+        pool.unused_(idx)
+    if res.converged:
+        if verboselevel >= 3:
+            print(f"Converged at {res.iterations}-th step. eps: {res.relative_residual}")
+        return None
+    raise RuntimeError(f"The CG is not converged! with maxsteps = {maxsteps}\nresidual is {res.relative_residual}\n"
+                       f"reason is {res.reason}\nmaxsteps should be larger.")
+
+
+def solve_(x, A: DdagDOp, b, r=None, p=None, Ap=None, verboselevel=2, **kw):
+    """solve!(x, A::DdagDOp, b, r, p, Ap; kwargs...) (DiracOperators.jl:124-132),
+    returning the CGResult; solve!(x, A::DdagDOp{<:DiracOp}, b) (:134-143), the
+    compatibility form without work fields, borrows them from A.D.phitemps
+    and returns None on convergence (raises otherwise)."""
+    if r is None and p is None and Ap is None:
+        if not (isinstance(A, DdagDOp) and isinstance(A.D, DiracOp)):
+            raise TypeError("solve_ without work fields needs a DdagDOp of a DiracOp")
+        return cg(x, A, b, A.D.phitemps, verboselevel=verboselevel)
     return cg_(x, A, b, r, p, Ap, **kw)
 
 
-def pseudofermion_action(D, phi, eta, Deta, r, p, Ap, **kw) -> float:
+def pseudofermion_action(D, phi, eta=None, Deta=None, r=None, p=None, Ap=None, **kw) -> float:
     """real(dot(phi, (D^dagger D)^-1 phi)); eta is the initial guess and the
-    solution (src/Operators/DiracOperators.jl:155-162)."""
+    solution (src/Operators/DiracOperators.jl:146-162).  pseudofermion_action(D,
+    phi) with a DiracOp borrows eta and the work fields from D.phitemps
+    (:164-180)."""
+    if eta is None and Deta is None and r is None and p is None and Ap is None:
+        if not isinstance(D, DiracOp):
+            raise TypeError("pseudofermion_action(D, phi) without work fields needs a DiracOp")
+        eta, i = D.phitemps.get_block()
+        try:
+            solve_(eta, DdagDOp(D), phi)
+            return dot(phi, eta).real
+        finally:
+            D.phitemps.unused_(i)
     res = solve_(eta, DdagDOp(D, Deta), phi, r, p, Ap, **kw)
     if not res.converged:
         raise RuntimeError(f"CG failed with reason {res.reason} after {res.iterations} iterations; "
