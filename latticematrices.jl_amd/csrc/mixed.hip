@@ -101,9 +101,9 @@ static size_t layout(lmx_context* ctx, int nrhs, int sp, MixPtrs* P, char* base)
 // of (a, b) over its 256 sites; a separate one-block-per-RHS finalize launch
 // sums them in a fixed order and updates that RHS's state.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void row_partials(double a, double b, double* partials) {
+__device__ __forceinline__ void row_partials(double a, double b, double* partials, int nb, int bx) {
     __shared__ double lds4[4];
-    const int nb = gridDim.x, i = blockIdx.y, bx = blockIdx.x;
+    const int i = blockIdx.y;
     double ta = block_tree(a, lds4);
     double tb = block_tree(b, lds4);
     if (threadIdx.x == 0) {
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, 
 #pragma unroll
         for (int m = 0; m < 6; ++m) xlo[(size_t)i * 6 * V + (size_t)m * V + s] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    row_partials(rr, bb, partials);
+    row_partials(rr, bb, partials, gridDim.x, blockIdx.x);
 }
 
 __global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta, int rmax) {
@@ -251,21 +251,15 @@ struct RingSlots {
     SI p[kXBatch - 1];
 };
 template <class SI>
-__global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float4* __restrict__ xlo, RingSlots<SI> ring,
-                                                  int V, const MixRhs* rhs, const MixBatch* batch, int force) {
-    const int i = blockIdx.y;
-    if ((!force && !batch->do_update) || rhs[i].done) return;
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= V) return;
-    const size_t o = (size_t)i * 12 * V, o4 = (size_t)i * 6 * V;
-    const int mask = rhs[i].pmask;  // slots not yet overwritten by a later p update
+__device__ __forceinline__ void accum_site(double2* __restrict__ x, float4* __restrict__ xlo, const RingSlots<SI>& ring,
+                                           int V, const MixRhs& st, int mask, size_t o, size_t o4, int i, int s) {
     double d[24];
 #pragma unroll
     for (int k = 0; k < 24; ++k) d[k] = 0.0;
 #pragma unroll
     for (int j = 0; j < kXBatch - 1; ++j) {
         if (mask & (1 << j)) {
-            const double ap = rhs[i].alpha_q[j];
+            const double ap = st.alpha_q[j];
             cx<float> q[12];
             ring.p[j].ld12(i, s, q);
 #pragma unroll
@@ -293,6 +287,19 @@ __global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float
     }
 }
 
+// (grid-stride over the sites: a small grid, so that the launch costs little
+// on the iterations where the update is gated off)
+template <class SI>
+__global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float4* __restrict__ xlo, RingSlots<SI> ring,
+                                                  int V, const MixRhs* rhs, const MixBatch* batch, int force) {
+    const int i = blockIdx.y;
+    if ((!force && !batch->do_update) || rhs[i].done) return;
+    const size_t o = (size_t)i * 12 * V, o4 = (size_t)i * 6 * V;
+    const int mask = rhs[i].pmask;  // slots not yet overwritten by a later p update
+    for (int s = blockIdx.x * 256 + threadIdx.x; s < V; s += gridDim.x * 256)
+        accum_site(x, xlo, ring, V, rhs[i], mask, o, o4, i, s);
+}
+
 // reliable update, part 2: r64 = b - A x (A x in r64), r_lo = r64, |r|^2 in FP64
 template <class SO>
 __global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__ b, double2* __restrict__ r64, SO rlo,
@@ -300,23 +307,29 @@ __global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__
                                                      const MixBatch* batch, int force) {
     const int i = blockIdx.y;
     if ((!force && !batch->do_update) || rhs[i].done) return;
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    double v = 0.0;
-    if (s < V) {
-        const size_t o = (size_t)i * 12 * V;
-        cx<float> f[12];
+    static_assert(kRedBlock == 256, "one partial per 256-site block");
+    const int nb = (V + 255) >> 8;
+    // grid-stride over the 256-site blocks (a small grid, cheap when gated
+    // off); one partial per block, as from a one-block-per-256-sites launch
+    for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+        const int s = blk * 256 + threadIdx.x;
+        double v = 0.0;
+        if (s < V) {
+            const size_t o = (size_t)i * 12 * V;
+            cx<float> f[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            const size_t e = o + (size_t)k * V + s;
-            double2 bv = b[e], av = r64[e];
-            double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
-            r64[e] = rv;
-            f[k] = mk<float>((float)rv.x, (float)rv.y);
-            v += rv.x * rv.x + rv.y * rv.y;
+            for (int k = 0; k < 12; ++k) {
+                const size_t e = o + (size_t)k * V + s;
+                double2 bv = b[e], av = r64[e];
+                double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
+                r64[e] = rv;
+                f[k] = mk<float>((float)rv.x, (float)rv.y);
+                v += rv.x * rv.x + rv.y * rv.y;
+            }
+            rlo.st12(i, s, f);
         }
-        rlo.st12(i, s, f);
+        row_partials(v, 0.0, partials, nb, blk);
     }
-    row_partials(v, 0.0, partials);
 }
 
 // p = r_lo + beta p
@@ -436,9 +449,14 @@ static int apply_ddag_xr(MixRun& R, int e) {
     return launch_wilson_lo(R.ctx, a);
 }
 
+// The reliable-update kernels run on most iterations only to find their gate
+// off; they stride over the lattice with at most this many workgroups per RHS.
+constexpr unsigned kUpdateBlocks = 256;
+
 template <class SO>
 static void enqueue_true_res(MixRun& R, dim3 grid, int force) {
     const int V = (int)R.ctx->V;
+    grid.x = std::min(grid.x, kUpdateBlocks);
     hipLaunchKernelGGL((k_mx_true_res<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V), V,
                        R.w.partials, R.w.rhs, R.w.batch, force);
 }
@@ -464,6 +482,7 @@ static RingSlots<SI> ring_of(MixRun& R) {
 }
 static void enqueue_accum(MixRun& R, dim3 grid, int force) {
     const int V = (int)R.ctx->V;
+    grid.x = std::min(grid.x, kUpdateBlocks);
     if (R.spinor_prec == LMX_H16)
         hipLaunchKernelGGL((k_mx_accum<SpH16In>), grid, dim3(256), 0, R.ctx->stream, R.x, R.w.xlo, ring_of<SpH16In>(R),
                            V, R.w.rhs, R.w.batch, force);
