@@ -54,14 +54,21 @@ def test_mixed_batched_converges_to_fp64_true_residual(gpu, precision, use_graph
         assert err <= 1e-7
 
 
-def test_graph_and_eager_agree(gpu):
-    L, kappa = (8, 4, 4, 8), 0.12
-    U, b, lat, Ug, D = setup(L, "haar", 310, kappa, nrhs=2)
+@pytest.mark.parametrize("precision,L,nrhs", [("f32", (8, 4, 4, 8), 2), ("h16", (8, 8, 4, 8), 5)])
+def test_graph_and_eager_agree(gpu, precision, L, nrhs):
+    # every solve replays its graph many times with host polls in between;
+    # the replays must give the eager launches' result bit for bit (8x8x4 is
+    # a streaming shape, 8x4x4 the per-site kernel)
+    kappa = 0.12
+    print(f"graph vs eager {precision} L={L} nrhs={nrhs}", flush=True)
+    U, b, lat, Ug, D = setup(L, "haar", 310, kappa, nrhs=nrhs)
     bg = lat.spinor_from_numpy(b)
-    x1, x2 = lat.spinor(2), lat.spinor(2)
-    r1 = lmx.cg_batched_(x1, D, bg, precision="f32", use_graph=False, check_every=5)
-    r2 = lmx.cg_batched_(x2, D, bg, precision="f32", use_graph=True, check_every=5)
+    x1, x2 = lat.spinor(nrhs), lat.spinor(nrhs)
+    r1 = lmx.cg_batched_(x1, D, bg, precision=precision, use_graph=False, check_every=5)
+    r2 = lmx.cg_batched_(x2, D, bg, precision=precision, use_graph=True, check_every=5)
+    assert all(r.converged for r in r1 + r2)
     assert [r.iterations for r in r1] == [r.iterations for r in r2]
+    assert [r.reliable_updates for r in r1] == [r.reliable_updates for r in r2]
     assert np.array_equal(x1.to_numpy(), x2.to_numpy())
 
 
