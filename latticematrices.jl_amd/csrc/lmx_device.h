@@ -283,9 +283,14 @@ constexpr float kH16Max = 32767.0f;
 
 __device__ __forceinline__ float h16_lo(int w) { return float(int(unsigned(w) << 16) >> 16); }
 __device__ __forceinline__ float h16_hi(int w) { return float(w >> 16); }
-__device__ __forceinline__ int h16_pack(float re, float im) {
-    const unsigned a = unsigned(__float2int_rn(re)), b = unsigned(__float2int_rn(im));
-    return int((a & 0xffffu) | (b << 16));
+// round(re * inv), round(im * inv) (|.| <= 32767, nearest-even, one rounding
+// of the exact product) as a 16-bit pair: x + 1.5 * 2^23 lies in [2^23, 2^24),
+// where the float spacing is 1, so the low 16 bits of its encoding are round(x)
+// in two's complement; one FMA per component and one byte permute per pair.
+__device__ __forceinline__ int h16_pack_scaled(float re, float im, float inv) {
+    constexpr float kMagic = 12582912.0f;
+    const float a = __builtin_fmaf(re, inv, kMagic), b = __builtin_fmaf(im, inv, kMagic);
+    return int(__builtin_amdgcn_perm(__float_as_uint(b), __float_as_uint(a), 0x05040100u));
 }
 
 struct SpH16In {
@@ -328,18 +333,19 @@ struct SpH16Out {
         float m = 0.0f;
 #pragma unroll
         for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(z[k].re), fabsf(z[k].im)));
-        const float inv = m > 0.0f ? kH16Max / m : 0.0f;
+        // 1 ulp reciprocal: m * inv stays within 32767 * (1 + 2^-22) < 32767.5
+        const float inv = m > 0.0f ? kH16Max * __builtin_amdgcn_rcpf(m) : 0.0f;
         int4* q = p + (size_t)r * 3 * V;
 #pragma unroll
         for (int g = 0; g < 3; ++g) {
             int4 v;
-            v.x = h16_pack(z[4 * g + 0].re * inv, z[4 * g + 0].im * inv);
-            v.y = h16_pack(z[4 * g + 1].re * inv, z[4 * g + 1].im * inv);
-            v.z = h16_pack(z[4 * g + 2].re * inv, z[4 * g + 2].im * inv);
-            v.w = h16_pack(z[4 * g + 3].re * inv, z[4 * g + 3].im * inv);
+            v.x = h16_pack_scaled(z[4 * g + 0].re, z[4 * g + 0].im, inv);
+            v.y = h16_pack_scaled(z[4 * g + 1].re, z[4 * g + 1].im, inv);
+            v.z = h16_pack_scaled(z[4 * g + 2].re, z[4 * g + 2].im, inv);
+            v.w = h16_pack_scaled(z[4 * g + 3].re, z[4 * g + 3].im, inv);
             st_out((int4*)((char*)(q + (size_t)g * V) + (unsigned)s * 16u), v);
         }
-        st_out((float*)((char*)(scale + (size_t)r * V) + (unsigned)s * 4u), m / kH16Max);
+        st_out((float*)((char*)(scale + (size_t)r * V) + (unsigned)s * 4u), m * (1.0f / kH16Max));
     }
 };
 
