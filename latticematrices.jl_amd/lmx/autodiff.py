@@ -109,3 +109,13 @@ def mul_cached_clover_ad(cache: WilsonDiracOperator4D, U: GaugeField, psi: Spino
     """D(U) psi through mul_cached_clover_ with autograd; U.data and psi.data
     may require grad.  Returns the result tensor [n, 12, V, 2]."""
     return MulCachedClover.apply(U.data, psi.data, cache, U)
+
+
+def wilson_apply_ad(D: WilsonDiracOperator4D, psi: SpinorField):
+    """D psi (mul!(result, D, psi)) for a Wilson or Wilson-clover operator
+    with autograd through its links (D.U.data) and psi.data: the reference's
+    mul! reverse rules (ext/AD/WilsonDiracOperator.jl:93-170,
+    ext/AD/WilsonDiracCloverOperator.jl:266-340) -- the analytic link pullback
+    and dpsi = D^dagger dresult -- through MulCachedClover with the operator's
+    own links (refreshing its derived copies first if they were written)."""
+    return MulCachedClover.apply(D.U.data, psi.data, D, D.U)

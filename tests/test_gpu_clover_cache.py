@@ -118,3 +118,23 @@ def test_cached_apply_autograd_is_the_pullback(gpu):
         worst = max(worst, abs(fd - g) / max(abs(g), 1.0))
     print(f"cached clover autograd: worst |FD - grad| / max(|grad|, 1) = {worst:.2e}", flush=True)
     assert worst < 1e-6
+
+
+@pytest.mark.parametrize("csw", [0.0, CSW])
+def test_operator_apply_autograd(gpu, csw):
+    # the mul! reverse rules of the Wilson and Wilson-clover operators
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    U = lat.gauge_from_numpy(gen.links(L, "haar", 930))
+    psi = lat.spinor_from_numpy(gen.gaussian_spinor(L, 931))
+    left = lat.spinor_from_numpy(gen.gaussian_spinor(L, 932))
+    links = U.data.clone().requires_grad_(True)
+    D = lmx.WilsonDiracOperator4D(lmx.GaugeField(lat, links), KAPPA, csw)
+    pg = lmx.SpinorField(lat, 1, psi.data.clone().requires_grad_(True))
+    out = lmx.wilson_apply_ad(D, pg)
+    (left.data * out).sum().backward()
+    ref = _fresh(lat, U, csw=csw)
+    assert torch.equal(out.detach(), _apply(ref, psi))
+    dU = lmx.GaugeField(lat)
+    lmx.wilson_pullback_(dU, ref, left, psi, -KAPPA)
+    assert torch.equal(links.grad, dU.data)
+    assert torch.equal(pg.data.grad, _apply(ref, left, dagger=True))
