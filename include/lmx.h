@@ -85,8 +85,9 @@ typedef struct {
     int32_t reliable_maxiter; /* ... or after this many inner iterations */
     int32_t check_every;      /* iterations between host convergence polls */
     int32_t use_graph;        /* capture check_every iterations in a hipGraph */
-    int32_t rhs_block;        /* inner dslash shape (0: autotuned): (TL << 12) | (RHS per launch << 4) | RHS per
-                                 work item (1, 2, 4); TL = t-slices per streaming workgroup, 0 automatic */
+    int32_t rhs_block;        /* inner dslash shape (0: autotuned): (ZS << 20) | (TL << 12) | (RHS per launch << 4)
+                                 | RHS per work item (1, 2, 4); TL = t-slices per streaming workgroup, ZS = 3-D
+                                 sub-ranges of the lattice (one XCD's slab spans one), 0 automatic for both */
     int32_t reserved[4];
 } lmx_cg_params;
 

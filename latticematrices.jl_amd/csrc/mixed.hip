@@ -796,18 +796,24 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     int rc = LMX_OK;
     best = 0;
     best_us = 1e30;
-    // t-slices per streaming workgroup: automatic, 4 or 8 (streaming lattices only)
-    const bool streams = ((long)c->g.L[0] * c->g.L[1] * c->g.L[2]) % 256 == 0;
+    // t-slices per streaming workgroup: automatic, 4 or 8; z-ranges per
+    // XCD slab: automatic, 1 or 2 (streaming lattices only)
+    const long V3 = (long)c->g.L[0] * c->g.L[1] * c->g.L[2];
+    const bool streams = V3 % 256 == 0;
     const int tl_opts[3] = {0, 4, 8};
+    const int zs_opts[3] = {0, 1, 2};
     for (int pi = 0; pi < 5 && rc == LMX_OK; ++pi) {
         const int per = per_opts[pi];
         if (per > nrhs) break;
         for (int ti = 0; ti < 3 && rc == LMX_OK; ++ti) {
         const int tl = tl_opts[ti];
         if (tl > 0 && (!streams || c->g.L[3] % tl != 0)) continue;
+        for (int zi = 0; zi < 3 && rc == LMX_OK; ++zi) {
+        const int zs = zs_opts[zi];
+        if (zs > 0 && (!streams || (V3 / 256) % zs != 0)) continue;
         for (int nr = 1; nr <= 4 && rc == LMX_OK; ++nr) {
             if (nr == 3 || nr > per || per % nr != 0) continue;  // built: 1, 2, 4 per work item
-            const int blk = (tl << 12) | (per << 4) | nr;
+            const int blk = (zs << 20) | (tl << 12) | (per << 4) | nr;
             ld.rhs_block = lx.rhs_block = blk;
             rc = group();
             const int reps = 2;
@@ -822,6 +828,7 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
                 best_us = us;
                 best = blk;
             }
+        }
         }
         }
     }

@@ -788,8 +788,8 @@ struct Shape {
 // t-streaming when the 3-D volume splits into 256-site blocks; TL and ZS keep
 // enough workgroups for the chip and keep one step's working set per XCD
 // small.  A context override (tune_tl / tune_zs) wins when it divides, then
-// the caller's (autotuned) hint.
-inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks, int tl_hint) {
+// the caller's (autotuned) hints.
+inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks, int tl_hint, int zs_hint = 0) {
     Shape sh{false, 1, 1};
     const Geom& g = ctx->g;
     const long V3 = (long)g.L[0] * g.L[1] * g.L[2];
@@ -812,6 +812,8 @@ inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks, int
     }
     if (ctx->tune_zs > 0 && nblk3 % ctx->tune_zs == 0) {
         sh.ZS = ctx->tune_zs;
+    } else if (zs_hint > 0 && nblk3 % zs_hint == 0) {
+        sh.ZS = zs_hint;
     } else {
         const int zs_cand[3] = {4, 2, 1};
         for (int z : zs_cand)
@@ -830,10 +832,11 @@ inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks, int
 // D^dagger, EPI_NORM on D.
 template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int NR, int EPI, bool SITE_NG, bool DONLY_OK>
 int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex, double kappa, double ccoef,
-             int ngroups, int nrhs, int flags, bool clover, const EpiArgs& epi, int min_blocks, int tl_hint = 0) {
+             int ngroups, int nrhs, int flags, bool clover, const EpiArgs& epi, int min_blocks, int tl_hint = 0,
+             int zs_hint = 0) {
     const bool dag = (flags & LMX_DAGGER) != 0;
     const bool donly = (flags & LMX_DONLY) != 0;
-    const Shape sh = pick_shape(ctx, ngroups, min_blocks, tl_hint);
+    const Shape sh = pick_shape(ctx, ngroups, min_blocks, tl_hint, zs_hint);
     const int nb_site = nblocks(ctx->V) * ngroups;
     const int nb_stream = (int)(ctx->V / 256 / sh.TL) * ngroups;
     const Geom& gg = ctx->g;
@@ -907,15 +910,17 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
 }
 
 // One low-precision (In, Out, Lk, Cl, NR) combination: (rhs_block >> 4) & 0xff
-// right-hand sides per launch, NR per work item, (rhs_block >> 12) t-slices
-// per streaming workgroup; a.epi selects the fused epilogue.  Instantiated
-// per combination in wilson_lo.hip.
+// right-hand sides per launch, NR per work item, (rhs_block >> 12) & 0xff
+// t-slices per streaming workgroup, (rhs_block >> 20) & 0xf z-ranges (the
+// 3-D share of one XCD's slab; 0 automatic); a.epi selects the fused
+// epilogue.  Instantiated per combination in wilson_lo.hip.
 template <class In, class Out, class Lk, class Cl, int NR>
 int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
     const double ccoef = (-a.kappa) * a.csw;
     int per_launch = (a.rhs_block >> 4) & 0xff;
     if (per_launch <= 0) per_launch = 1;
     const int tl = (a.rhs_block >> 12) & 0xff;
+    const int zs = (a.rhs_block >> 20) & 0xf;
     const int V = (int)ctx->V;
     for (int r0 = 0; r0 < a.nrhs; r0 += per_launch) {
         const int n = std::min(per_launch, a.nrhs - r0);
@@ -941,13 +946,13 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
             for (int j = 0; j < kXBatch - 1; ++j)
                 ex.pq[j] = In::at(a.pq[j] ? a.pq[j] : a.aux, a.pq[j] ? a.pq_norm[j] : a.aux_norm, V, r0);
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, XrEx<In, Out>, NR, EPI_XR, false, false>(
-                ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl)));
+                ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
         } else if (a.epi == EPI_NORM) {
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, NoEx, NR, EPI_NORM, false, false>(
-                ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl)));
+                ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
         } else if (a.epi == EPI_NONE) {
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, NoEx, NR, EPI_NONE, false, false>(
-                ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl)));
+                ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
         } else {
             set_error("fused epilogue %d is not built for low-precision spinors", a.epi);
             return LMX_ERR_UNSUPPORTED;

@@ -206,3 +206,22 @@ def test_rhs_count_limits(gpu):
     b65 = lat.spinor(65)
     with pytest.raises(lmx.ArgumentError):
         lmx.cg_batched_(lat.spinor(65), D, b65, precision="h16")
+
+
+def test_z_range_shapes_are_bit_identical(gpu):
+    # the z-range count (rhs_block bits 20-23) only changes which workgroup
+    # computes which site; arithmetic and reduction order are the same
+    L, kappa = (16, 8, 8, 8), 0.12
+    print(f"z-range shapes L={L}", flush=True)
+    U, b, lat, Ug, D = setup(L, "weak", 320, kappa, nrhs=3)
+    bg = lat.spinor_from_numpy(b)
+    ref = None
+    for zs in (1, 2, 4):
+        x = lat.spinor(3)
+        res = lmx.cg_batched_(x, D, bg, precision="h16", use_graph=True, rhs_block=(zs << 20) | (4 << 12) | (3 << 4) | 1)
+        assert all(r.converged for r in res)
+        got = ([r.iterations for r in res], x.to_numpy())
+        if ref is None:
+            ref = got
+        else:
+            assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), zs
