@@ -249,11 +249,15 @@ class WilsonDiracOperator4D:
 
     def clover_planes(self):
         """The six clover field-strength planes [6, V, 3, 3] (CloverFieldStrength4D)."""
+        return self.clover_planes_device().cpu().numpy()
+
+    def clover_planes_device(self):
+        """The same planes as a complex128 device tensor [6, V, 3, 3] (plane, site, row, col)."""
         L = self.lattice
-        out = torch.empty(6 * L.V * 9, dtype=torch.complex128, device=L.device)
+        out = torch.empty((6, L.V, 3, 3), dtype=torch.complex128, device=L.device)
         check(lib().lmx_clover_export(self._op, ctypes.c_void_p(_ptr(out))))
         L.synchronize()
-        return out.cpu().numpy().reshape(6, L.V, 3, 3)
+        return out
 
     def __del__(self):
         try:
@@ -268,6 +272,47 @@ def WilsonDiracCloverOperator4D(U: GaugeField, kappa: float, cSW: float) -> Wils
     """D = D_W - kappa cSW sum_{mu<nu} gamma_mu gamma_nu F_mu,nu
     (src/Operators/WilsonDiracCloverOperator.jl:403-456)."""
     return WilsonDiracOperator4D(U, kappa, cSW)
+
+
+class CloverFieldStrength4D:
+    """CloverFieldStrength4D(U) (src/Operators/WilsonDiracCloverOperator.jl:1-33,
+    386-399): the six planes (12, 13, 14, 23, 24, 34) of F = (Q - Q^dagger)/8,
+    Q the four-leaf plaquette sum, built on the device by the operator's
+    clover kernel.  len(F) == 6; F[i] is a complex128 device tensor
+    [V, 3, 3] (site, row, col)."""
+
+    def __init__(self, U: GaugeField):
+        self.lattice = U.lattice
+        self.components = ()
+        update_clover_(self, U)
+
+    def __len__(self):
+        return 6
+
+    def __getitem__(self, i):
+        return self.components[i]
+
+
+def update_clover_(target, U: GaugeField | None = None):
+    """update_clover!(F::CloverFieldStrength4D, U) recomputes the six planes
+    from U (src/Operators/WilsonDiracCloverOperator.jl:392-399);
+    update_clover!(operator[, U]) refreshes an operator's cache, optionally
+    from new links (:462-479)."""
+    if isinstance(target, CloverFieldStrength4D):
+        if not isinstance(U, GaugeField):
+            raise TypeError("update_clover_(F, U) needs the links U as a GaugeField")
+        if U.lattice.dims != target.lattice.dims:
+            raise ArgumentError(_lib.LMX_ERR_ARG, "links of another lattice geometry")
+        op = WilsonDiracOperator4D(U, 0.125, 1.0)  # any nonzero cSW builds the field strength
+        planes = op.clover_planes_device()
+        del op
+        target.components = tuple(planes[i] for i in range(6))
+        return target
+    if isinstance(target, WilsonDiracOperator4D):
+        target.update_links(U)
+        return target
+    raise TypeError(f"update_clover_ needs a CloverFieldStrength4D or a Wilson-clover operator, not "
+                    f"{type(target).__name__}")
 
 
 def _ensure_clover_cache_current(cache: WilsonDiracOperator4D, U: GaugeField) -> WilsonDiracOperator4D:

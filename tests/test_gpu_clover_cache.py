@@ -138,3 +138,25 @@ def test_operator_apply_autograd(gpu, csw):
     lmx.wilson_pullback_(dU, ref, left, psi, -KAPPA)
     assert torch.equal(links.grad, dU.data)
     assert torch.equal(pg.data.grad, _apply(ref, left, dagger=True))
+
+
+def test_clover_field_strength(gpu):
+    # CloverFieldStrength4D(U) and update_clover!(F, U): the oracle's planes
+    # bit for bit (as the operator's cache, test_gpu_parity), and refreshed
+    import oracle as orc
+    Lf = (4, 4, 2, 4)
+    lat = lmx.Lattice(Lf, (1, 1, 1, -1))
+    U1, U2 = gen.links(Lf, "weak", 940, eps=0.15), gen.links(Lf, "haar", 941)
+    F = lmx.CloverFieldStrength4D(lat.gauge_from_numpy(U1))
+    assert len(F) == 6
+    for U in (U1, U2):
+        if U is U2:
+            assert lmx.update_clover_(F, lat.gauge_from_numpy(U2)) is F
+        want = orc.clover_field_strength(U, Lf)
+        got = np.stack([F[i].cpu().numpy() for i in range(6)])
+        assert np.array_equal(got, want)
+        assert torch.allclose(F[3], -F[3].conj().transpose(-1, -2), rtol=0, atol=1e-15)  # anti-Hermitian
+    with pytest.raises(TypeError):
+        lmx.update_clover_(F, U2)
+    with pytest.raises(TypeError):
+        lmx.update_clover_(object(), None)
