@@ -195,6 +195,12 @@ int lmx_clover_export(lmx_wilson* op, void* F_out);
  * (src/Operators/WilsonDiracOperator.jl:60-76,425-441,535-551,646-662) and the
  * clover mul! (src/Operators/WilsonDiracCloverOperator.jl:561-601). */
 int lmx_wilson_apply(lmx_wilson* op, void* out, const void* in, int flags, int nrhs);
+/* out = D in in single precision: mul!(C, D, psi) on ComplexF32 fields
+ * (LatticeMatrix{...,ComplexF32}, test/wilson_halfspin.jl:105,124-130).  out
+ * and in are float2 [nrhs][12][V] (component spin*3+colour); the operator's
+ * links and clover cache are used rounded to FP32 and the arithmetic is
+ * FP32.  Flags as lmx_wilson_apply (LMX_DAGGER, LMX_DONLY). */
+int lmx_wilson_apply_f32(lmx_wilson* op, void* out, const void* in, int flags, int nrhs);
 /* The same mul! directly on the reference's storage, without conversion:
  * psi and C are LatticeMatrix.A arrays (complex128, column-major
  * [3,4,L1+2nw,L2+2nw,L3+2nw,L4+2nw], nrhs of them stacked).  psi's halo is

@@ -401,6 +401,31 @@ int lmx_wilson_apply(lmx_wilson* op, void* out, const void* in, int flags, int n
                              (double2*)out, (const double2*)in, nrhs, flags, e);
 }
 
+int lmx_wilson_apply_f32(lmx_wilson* op, void* out, const void* in, int flags, int nrhs) {
+    clear_error();
+    LMX_TRY(check_op(op));
+    LMX_CHECK(out && in && nrhs >= 0, LMX_ERR_ARG, "bad wilson_apply_f32 arguments");
+    LMX_CHECK(out != in, LMX_ERR_ALIAS, "output and input of the Wilson operator must not alias");
+    LMX_CHECK((flags & ~(LMX_DAGGER | LMX_DONLY)) == 0, LMX_ERR_ARG, "unknown flags %d", flags);
+    if (nrhs == 0 || op->ctx->V == 0) return LMX_OK;
+    LMX_TRY(ensure_lo_links(op));
+    LoArgs a;
+    memset(&a, 0, sizeof(a));
+    a.link_prec = LMX_F32;
+    a.spinor_prec = LMX_F32;
+    a.links = op->links32;
+    a.clover = (flags & LMX_DONLY) ? nullptr : op->clover32;
+    a.kappa = op->kappa;
+    a.csw = op->csw;
+    a.out = out;
+    a.in = in;
+    a.nrhs = nrhs;
+    a.flags = flags;
+    a.epi = 0;
+    a.rhs_block = (std::min(nrhs, 12) << 4) | 1;  // one RHS per work item, up to 12 per launch
+    return launch_wilson_lo(op->ctx, a);
+}
+
 int lmx_wilson_apply_padded(lmx_wilson* op, void* C, const void* psi, int nw, int flags, int nrhs) {
     clear_error();
     LMX_TRY(check_op(op));

@@ -951,7 +951,10 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, NoEx, NR, EPI_NORM, false, false>(
                 ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
         } else if (a.epi == EPI_NONE) {
-            LMX_TRY((dispatch<In, Lk, Out, Cl, In, NoEx, NR, EPI_NONE, false, false>(
+            // the single-precision user apply (FP32 spinors and links) also
+            // serves the hopping-only operator
+            constexpr bool kDonly = std::is_same<In, SpF32In>::value && std::is_same<Lk, LinkF32>::value && NR == 1;
+            LMX_TRY((dispatch<In, Lk, Out, Cl, In, NoEx, NR, EPI_NONE, false, kDonly>(
                 ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
         } else {
             set_error("fused epilogue %d is not built for low-precision spinors", a.epi);

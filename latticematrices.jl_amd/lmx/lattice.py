@@ -505,6 +505,51 @@ def mul_(C: SpinorField, A, psi: SpinorField) -> SpinorField:
     return C
 
 
+def spinor_f32(lattice: Lattice, nrhs: int = 1):
+    """A zero single-precision spinor field: float32 device tensor [nrhs, 12, V, 2]
+    (component spin*3+colour, (re, im)) -- LatticeMatrix{ComplexF32} in the
+    library's layout."""
+    return torch.zeros((nrhs, 12, lattice.V, 2), dtype=torch.float32, device=lattice.device)
+
+
+def spinor_f32_from_numpy(lattice: Lattice, arr):
+    """[V, 4, 3] or [nrhs, V, 4, 3] complex (site, spin, colour) -> float32 [nrhs, 12, V, 2]."""
+    a = np.asarray(arr, dtype=np.complex64)
+    if a.ndim == 3:
+        a = a[None]
+    if a.shape[1:] != (lattice.V, 4, 3):
+        raise ArgumentError(_lib.LMX_ERR_ARG, f"spinor array of shape {a.shape} does not match V = {lattice.V}")
+    a = np.ascontiguousarray(a.reshape(a.shape[0], lattice.V, 12).transpose(0, 2, 1))
+    return torch.view_as_real(torch.from_numpy(a)).contiguous().to(lattice.device)
+
+
+def spinor_f32_to_numpy(t):
+    """float32 [nrhs, 12, V, 2] -> complex64 [nrhs, V, 4, 3]."""
+    c = torch.view_as_complex(t.detach().contiguous()).cpu().numpy()
+    return np.ascontiguousarray(c.transpose(0, 2, 1)).reshape(c.shape[0], c.shape[2], 4, 3)
+
+
+def mul_f32_(C, A, psi):
+    """mul!(C, A, psi) on single-precision fields (ComplexF32,
+    test/wilson_halfspin.jl:105,124-130): C and psi are float32 device tensors
+    [nrhs, 12, V, 2] (spinor_f32); A is a Wilson(-clover) operator, its
+    adjoint, or the hopping-only operator.  FP32 links (the operator's,
+    rounded) and FP32 arithmetic."""
+    h, flags = _wilson_handle(A)
+    for t in (C, psi):
+        if not (isinstance(t, torch.Tensor) and t.dtype == torch.float32 and t.is_contiguous() and t.dim() == 4
+                and t.shape[1:] == (12, t.shape[2], 2)):
+            raise ArgumentError(_lib.LMX_ERR_ARG, "single-precision fields must be contiguous float32 [nrhs, 12, V, 2]")
+    if C.shape != psi.shape:
+        raise ArgumentError(_lib.LMX_ERR_ARG, "C and psi must have the same shape")
+    op = A.parent if isinstance(A, Adjoint) else A
+    lat = op.lattice
+    if C.shape[2] != lat.V or C.device != lat.device:
+        raise ArgumentError(_lib.LMX_ERR_ARG, "fields do not match the operator's lattice")
+    check(lib().lmx_wilson_apply_f32(h, ctypes.c_void_p(_ptr(C)), ctypes.c_void_p(_ptr(psi)), flags, C.shape[0]))
+    return C
+
+
 def mul_padded_(C, A, psi, nw: int):
     """mul!(C, A, psi) on arrays in the reference's own storage (no
     conversion): C and psi are device complex128 tensors laid out as

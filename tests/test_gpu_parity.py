@@ -423,3 +423,56 @@ def test_multi_rhs_fp64_apply_bit_exact(gpu, L, phases, nrhs):
             y1 = lat.spinor()
             lmx.mul_(y1, lmx.adjoint(Dc) if dag else Dc, x.rhs(r))
             assert np.array_equal(got[r], y1.to_numpy()), ("clover", dag, r)
+
+
+F32_CASES = [
+    ((4, 2, 3, 2), "rand", (1, 1, 1, -1)),
+    ((3, 2, 2, 4), "rand", (np.exp(0.3j), 1, -1j, -1)),  # complex phases: per-site kernel
+    ((8, 8, 4, 8), "haar", (-1, 1, 1, -1)),              # streaming kernel
+]
+
+
+@pytest.mark.parametrize("L,kind,phases", F32_CASES)
+@pytest.mark.parametrize("csw", [0.0, 1.3])
+def test_single_precision_apply(gpu, L, kind, phases, csw):
+    # test/wilson_halfspin.jl:105-121 at ComplexF32: D, D^dagger, the
+    # hopping-only operator and its adjoint, and the clover operator within
+    # 2e-5 of the FP64 oracle, and <chi, D psi> = <D^dagger chi, psi>
+    kappa, tol = 0.117, 2e-5
+    print(f"FP32 apply L={L} kind={kind} csw={csw}", flush=True)
+    U = make_links(L, kind, 11)
+    V = int(np.prod(L))
+    psi = rand_c(np.random.default_rng(12), (V, 4, 3))
+    chi = rand_c(np.random.default_rng(13), (V, 4, 3))
+    lat = lmx.Lattice(L, phases)
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracOperator4D(Ug, kappa, csw)
+    F = orc.clover_field_strength(U, L) if csw else None
+    x = lmx.spinor_f32_from_numpy(lat, psi)
+    y = lmx.spinor_f32(lat)
+    worst = 0.0
+    for flags in (0, 1, 2, 3):
+        A = lmx.WilsonDiracOperator4D_Donly(Ug) if flags & 2 else D
+        if flags & 1:
+            A = lmx.adjoint(A)
+        lmx.mul_f32_(y, A, x)
+        got = lmx.spinor_f32_to_numpy(y)[0]
+        if csw and not flags & 2:
+            ref = orc.wilson_clover(U, F, psi, L, kappa, csw, phases, dagger=bool(flags & 1))
+        else:
+            ref = orc.wilson(U, psi, L, kappa, phases, dagger=bool(flags & 1), donly=bool(flags & 2))
+        e = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        worst = max(worst, e)
+        assert e <= tol, (flags, e)
+    # adjoint identity in FP32
+    yc = lmx.spinor_f32(lat)
+    lmx.mul_f32_(y, D, x)
+    lmx.mul_f32_(yc, lmx.adjoint(D), lmx.spinor_f32_from_numpy(lat, chi))
+    lhs = np.vdot(chi.astype(np.complex64), lmx.spinor_f32_to_numpy(y)[0])
+    rhs = np.vdot(lmx.spinor_f32_to_numpy(yc)[0], psi.astype(np.complex64))
+    assert abs(lhs - rhs) <= tol * abs(lhs)
+    print(f"  worst relative error vs FP64 oracle {worst:.2e}", flush=True)
+    with pytest.raises(lmx.ArgumentError):
+        lmx.mul_f32_(y, D, y)  # aliasing
+    with pytest.raises(lmx.ArgumentError):
+        lmx.mul_f32_(y, D, lmx.spinor_f32(lat).double().float()[:, :6].contiguous())
