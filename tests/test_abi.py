@@ -44,6 +44,10 @@ def test_abi_version_and_errors_without_device_calls():
     assert b"positive" in L.lmx_last_error()
     p = _lib.default_params()
     assert p.rtol == 1e-10 and p.atol == 0.0 and p.maxiter == 5000
+    # operator entry points reject a NULL operator before touching the device
+    for name in ("lmx_wilson_apply", "lmx_wilson_apply_f32"):
+        rc = getattr(L, name)(None, None, None, 0, 1)
+        assert rc != _lib.LMX_OK, name
 
 
 C_CONSUMER = r"""

@@ -5,6 +5,7 @@ stencil, clover, BLAS-1 and CG kernels evaluate the reference's operations in
 the reference's order, so they are also checked for bit equality."""
 import numpy as np
 import pytest
+import torch
 
 import dense_reference as dr
 import oracle as orc
@@ -476,3 +477,20 @@ def test_single_precision_apply(gpu, L, kind, phases, csw):
         lmx.mul_f32_(y, D, y)  # aliasing
     with pytest.raises(lmx.ArgumentError):
         lmx.mul_f32_(y, D, lmx.spinor_f32(lat).double().float()[:, :6].contiguous())
+
+
+def test_single_precision_apply_many_rhs(gpu):
+    # several fields in one call equal the fields applied one by one
+    L = (8, 8, 4, 8)
+    U = make_links(L, "haar", 21)
+    V = int(np.prod(L))
+    psi = rand_c(np.random.default_rng(22), (3, V, 4, 3))
+    lat = lmx.Lattice(L, (1, 1, 1, -1))
+    D = lmx.WilsonDiracCloverOperator4D(lat.gauge_from_numpy(U), 0.12, 1.1)
+    x = lmx.spinor_f32_from_numpy(lat, psi)
+    y = lmx.spinor_f32(lat, 3)
+    lmx.mul_f32_(y, lmx.adjoint(D), x)
+    for r in range(3):
+        y1 = lmx.spinor_f32(lat)
+        lmx.mul_f32_(y1, lmx.adjoint(D), x[r:r + 1].contiguous())
+        assert torch.equal(y[r:r + 1], y1)
