@@ -3,51 +3,51 @@ reference's golden vectors (test/site_rng.jl), Random123's and PCG's published
 known answers, closed-form identities and the reference's statistical and
 fill tests (test/random_fill.jl)."""
 import itertools
+import json
+import os
 
 import numpy as np
 import pytest
 
 import oracle as orc
 
-KEY = (0x123456789ABCDEF0, 17, 3, 1, 2)  # test/site_rng.jl:14
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "rng_known_answers.json")))
+KEY = tuple(GOLDEN["site_rng"]["key"])  # test/site_rng.jl:14
 ALGS = ("pcg32", "xoshiro256++", "philox4x32")
 
 
 def test_philox4x32_10_known_answers():
     # Random123 kat_vectors for philox4x32_10; the first is also test/site_rng.jl:136-145
-    kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
-           ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
-           ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
-            (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
-    for ctr, key, out in kat:
-        assert orc.philox4x32_10(ctr, key) == out
+    for v in GOLDEN["philox4x32_10_kat"]["vectors"]:
+        assert orc.philox4x32_10(tuple(v["ctr"]), tuple(v["key"])) == tuple(v["out"])
 
 
 def test_pcg32_step_reference_sequence():
     # pcg32_srandom_r(rng, 42, 54) then six draws: the PCG reference demo
-    inc = (54 << 1) | 1
+    demo = GOLDEN["pcg32_demo"]
+    inc = (demo["stream"] << 1) | 1
     st, _ = orc.pcg32_step(0, inc)
-    st, _ = orc.pcg32_step((st + 42) % (1 << 64), inc)
+    st, _ = orc.pcg32_step((st + demo["seed"]) % (1 << 64), inc)
     out = []
     for _ in range(6):
         st, v = orc.pcg32_step(st, inc)
         out.append(v)
-    assert out == [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+    assert out == demo["out"]
 
 
 def test_site_stream_golden_words():
-    pcg = orc.site_rng_draw(KEY, "pcg32", 41, 1, words=8)["words"][0]  # test/site_rng.jl:23-37
-    assert list(pcg) == [0xF6BB5D81, 0x0DA05710, 0xE9BEAE27, 0x595296CB, 0xCB1DFF86, 0xF6F95CDC, 0x430127FB,
-                         0x51F359DA]
-    philox = orc.site_rng_draw(KEY, "philox4x32", 41, 1, words=8)["words"][0]  # :146-160
-    assert list(philox) == [0xF03945A7, 0xBBCBC866, 0xB9B8C5FD, 0xA4C43EF5, 0x2344CF49, 0xFD2FF13D, 0xDD20337F,
-                            0x22EBFDF9]
+    gs = GOLDEN["site_rng"]
+    site = gs["site"]
+    pcg = orc.site_rng_draw(KEY, "pcg32", site, 1, words=8)["words"][0]  # test/site_rng.jl:23-37
+    assert list(pcg) == gs["pcg32_u32"]
+    philox = orc.site_rng_draw(KEY, "philox4x32", site, 1, words=8)["words"][0]  # :146-160
+    assert list(philox) == gs["philox4x32_u32"]
     # xoshiro256++ u64 words (:120-134): u32 draws are their high halves and
     # Float64 uniforms their top 53 bits
-    x64 = [0x6AE4378D9CFD95CB, 0xEC28CE8E467F5024, 0x9A85A2BCB03F1F8A, 0x7FDD12651519F8A9]
-    xw = orc.site_rng_draw(KEY, "xoshiro256++", 41, 1, words=4)["words"][0]
+    x64 = gs["xoshiro256pp_u64"]
+    xw = orc.site_rng_draw(KEY, "xoshiro256++", site, 1, words=4)["words"][0]
     assert list(xw) == [v >> 32 for v in x64]
-    xu = orc.site_rng_draw(KEY, "xoshiro256++", 41, 1, uniform64=4)["uniform64"][0]
+    xu = orc.site_rng_draw(KEY, "xoshiro256++", site, 1, uniform64=4)["uniform64"][0]
     assert [int(u * 2.0 ** 53) for u in xu] == [v >> 11 for v in x64]
 
 
