@@ -851,7 +851,7 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
         return LMX_ERR_UNSUPPORTED;
     }
     if (donly && !DONLY_OK) {
-        set_error("hopping-only operator is built in FP64 only");
+        set_error("hopping-only operator is built for FP64 and FP32 fields only");
         return LMX_ERR_UNSUPPORTED;
     }
 #define LMX_GO(FPM, MODE)                                                                                      \
