@@ -212,6 +212,10 @@ int lmx_wilson_apply_f32(lmx_wilson* op, void* out, const void* in, int flags, i
  * (src/Operators/WilsonDiracOperator.jl:60-76,425-441; clover
  * src/Operators/WilsonDiracCloverOperator.jl:561-601). */
 int lmx_wilson_apply_padded(lmx_wilson* op, void* C, const void* psi, int nw, int flags, int nrhs);
+/* The same on ComplexF32 storage (LatticeMatrix{...,ComplexF32}.A, complex64,
+ * column-major [3,4,L1+2nw,...]): FP32 links (the operator's, rounded) and
+ * FP32 arithmetic, as lmx_wilson_apply_f32. */
+int lmx_wilson_apply_padded_f32(lmx_wilson* op, void* C, const void* psi, int nw, int flags, int nrhs);
 /* out = D^dagger D in with caller temp -- DdagDOp(D, temp) mul!,
  * src/Operators/DiracOperators.jl:104-121 (temp must not alias in/out). */
 int lmx_ddagd_apply(lmx_wilson* op, void* out, const void* in, void* tmp, int nrhs);

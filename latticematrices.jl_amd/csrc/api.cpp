@@ -436,6 +436,18 @@ int lmx_wilson_apply_padded(lmx_wilson* op, void* C, const void* psi, int nw, in
                                 (double2*)C, (const double2*)psi, nw, nrhs, flags);
 }
 
+int lmx_wilson_apply_padded_f32(lmx_wilson* op, void* C, const void* psi, int nw, int flags, int nrhs) {
+    clear_error();
+    LMX_TRY(check_op(op));
+    LMX_CHECK(C && psi && nrhs >= 0 && nw >= 0, LMX_ERR_ARG, "bad wilson_apply_padded_f32 arguments");
+    LMX_CHECK(C != psi, LMX_ERR_ALIAS, "output and input of the Wilson operator must not alias");
+    LMX_CHECK((flags & ~(LMX_DAGGER | LMX_DONLY)) == 0, LMX_ERR_ARG, "unknown flags %d", flags);
+    if (nrhs == 0 || op->ctx->V == 0) return LMX_OK;
+    LMX_TRY(ensure_lo_links(op));
+    return launch_wilson_padded_f32(op->ctx, op->links32, (flags & LMX_DONLY) ? nullptr : op->clover32, op->kappa,
+                                    op->csw, (float2*)C, (const float2*)psi, nw, nrhs, flags);
+}
+
 int lmx_ddagd_apply(lmx_wilson* op, void* out, const void* in, void* tmp, int nrhs) {
     clear_error();
     LMX_TRY(check_op(op));

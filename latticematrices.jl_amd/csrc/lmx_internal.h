@@ -109,6 +109,8 @@ int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover,
 // mul! on the reference's halo-padded AoS spinor arrays (halo ignored, core written)
 int launch_wilson_padded(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
                          double2* C, const double2* psi, int nw, int nrhs, int flags);
+int launch_wilson_padded_f32(lmx_context* ctx, const float2* U, const float2* clover, double kappa, double csw,
+                             float2* C, const float2* psi, int nw, int nrhs, int flags);
 
 // Low-precision apply: spinors LMX_F32 or LMX_H16 (data + per-site scale),
 // links LMX_F32 or LMX_F16, FP32 arithmetic.  epi selects a fused epilogue

@@ -94,9 +94,9 @@ struct PadGeom {
     long Vp;    // padded volume
 };
 
-template <int MU, int PM, bool FWD, typename R>
-__device__ __forceinline__ void hop_pad(cx<R> (&acc)[12], const double2* __restrict__ q, const LinkF64& U,
-                                        const Geom& g, const PadGeom& pg, int s, long sp, const int x[4]) {
+template <int MU, int PM, bool FWD, typename R, typename E, class Lk>
+__device__ __forceinline__ void hop_pad(cx<R> (&acc)[12], const E* __restrict__ q, const Lk& U, const Geom& g,
+                                        const PadGeom& pg, int s, long sp, const int x[4]) {
     bool w;
     const int t = FWD ? nbr_fwd(s, x, MU, g, w) : nbr_bwd(s, x, MU, g, w);
     const long step = (long)pg.PS[MU];
@@ -106,7 +106,7 @@ __device__ __forceinline__ void hop_pad(cx<R> (&acc)[12], const double2* __restr
     U.template ld9<R>(MU, FWD ? s : t, u);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-        double2 v = q[tp * 12 + k];
+        const E v = q[tp * 12 + k];
         psi[k] = mk<R>(v.x, v.y);
     }
     if (w && !g.phase_one[MU]) {
@@ -124,22 +124,25 @@ __device__ __forceinline__ void hop_pad(cx<R> (&acc)[12], const double2* __restr
     pin_acc(acc);
 }
 
-template <int FPM, int MODE>
-__global__ void __launch_bounds__(256) k_wilson_pad(double2* __restrict__ C, const double2* __restrict__ psi,
-                                                    LinkF64 U, F64Clover F, Geom g, PadGeom pg, double kappa,
-                                                    double ccoef, int nrhs) {
+// E: the field element (double2 for ComplexF64, float2 for ComplexF32); the
+// arithmetic is in E's precision, links Lk and clover Cl of the same.
+template <int FPM, int MODE, typename E, class Lk, class Cl>
+__global__ void __launch_bounds__(256) k_wilson_pad(E* __restrict__ C, const E* __restrict__ psi, Lk U, Cl F, Geom g,
+                                                    PadGeom pg, double kappa_d, double ccoef_d, int nrhs) {
+    using R = typename std::conditional<std::is_same<E, float2>::value, float, double>::type;
     const long gid = (long)blockIdx.x * 256 + threadIdx.x;
     if (gid >= (long)g.V * nrhs) return;
     const int s = (int)(gid % g.V);
     const int r = (int)(gid / g.V);
+    const R kappa = R(kappa_d), ccoef = R(ccoef_d);
     int x[4];
     site_coords(s, g, x);
     const long sp = (long)(x[0] + pg.nw) + (long)pg.PS[1] * (x[1] + pg.nw) + (long)pg.PS[2] * (x[2] + pg.nw) +
                     (long)pg.PS[3] * (x[3] + pg.nw);
-    const double2* q = psi + (long)r * pg.Vp * 12;
-    cx<double> acc[12];
+    const E* q = psi + (long)r * pg.Vp * 12;
+    cx<R> acc[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k] = mk<double>(0.0, 0.0);
+    for (int k = 0; k < 12; ++k) acc[k] = mk<R>(R(0), R(0));
     hop_pad<0, FPM, true>(acc, q, U, g, pg, s, sp, x);
     hop_pad<0, -FPM, false>(acc, q, U, g, pg, s, sp, x);
     hop_pad<1, FPM, true>(acc, q, U, g, pg, s, sp, x);
@@ -148,32 +151,39 @@ __global__ void __launch_bounds__(256) k_wilson_pad(double2* __restrict__ C, con
     hop_pad<2, -FPM, false>(acc, q, U, g, pg, s, sp, x);
     hop_pad<3, FPM, true>(acc, q, U, g, pg, s, sp, x);
     hop_pad<3, -FPM, false>(acc, q, U, g, pg, s, sp, x);
-    cx<double> res[12];
+    cx<R> res[12];
     if (MODE == MODE_DONLY) {
 #pragma unroll
         for (int sp4 = 0; sp4 < 4; ++sp4)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) res[sp4 * 3 + c] = cadd(mk<double>(0.0, 0.0), rmul(0.5, acc[c * 4 + sp4]));
+            for (int c = 0; c < 3; ++c)
+                res[sp4 * 3 + c] = cadd(mk<R>(R(0), R(0)), rmul(R(0.5), acc[c * 4 + sp4]));
     } else {
-        cx<double> p[12];
+        cx<R> p[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            double2 v = q[sp * 12 + k];
-            p[k] = mk<double>(v.x, v.y);
+            const E v = q[sp * 12 + k];
+            p[k] = mk<R>(v.x, v.y);
         }
 #pragma unroll
         for (int sp4 = 0; sp4 < 4; ++sp4)
 #pragma unroll
             for (int c = 0; c < 3; ++c) res[sp4 * 3 + c] = cadd(p[sp4 * 3 + c], rmul(-kappa, acc[c * 4 + sp4]));
-        if constexpr (MODE == MODE_CLOVER) clover_add<double>(F, s, g.V, p, ccoef, res);
+        if constexpr (MODE == MODE_CLOVER) clover_add<R>(F, s, g.V, p, ccoef, res);
     }
-    double2* o = C + (long)r * pg.Vp * 12 + sp * 12;
+    E* o = C + (long)r * pg.Vp * 12 + sp * 12;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) o[k] = make_double2(res[k].re, res[k].im);
+    for (int k = 0; k < 12; ++k) {
+        E v;
+        v.x = res[k].re;
+        v.y = res[k].im;
+        o[k] = v;
+    }
 }
 
-int launch_wilson_padded(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
-                         double2* C, const double2* psi, int nw, int nrhs, int flags) {
+template <typename E, class Lk, class Cl>
+static int launch_padded(lmx_context* ctx, Lk l, Cl F, bool clover, double kappa, double csw, E* C, const E* psi,
+                         int nw, int nrhs, int flags) {
     if (ctx->V == 0 || nrhs == 0) return LMX_OK;
     const Geom& g = ctx->g;
     PadGeom pg;
@@ -188,11 +198,10 @@ int launch_wilson_padded(lmx_context* ctx, const double2* U, const double2* clov
     const double ccoef = (-kappa) * csw;
     const long n = ctx->V * (long)nrhs;
     dim3 grid((unsigned)((n + 255) / 256)), blk(256);
-    LinkF64 l{U, (int)ctx->V};
-    F64Clover F{clover};
     const bool dag = (flags & LMX_DAGGER) != 0, donly = (flags & LMX_DONLY) != 0;
-#define LMX_PAD(FPM, MODE) \
-    hipLaunchKernelGGL((k_wilson_pad<FPM, MODE>), grid, blk, 0, ctx->stream, C, psi, l, F, g, pg, kappa, ccoef, nrhs)
+#define LMX_PAD(FPM, MODE)                                                                                    \
+    hipLaunchKernelGGL((k_wilson_pad<FPM, MODE, E, Lk, Cl>), grid, blk, 0, ctx->stream, C, psi, l, F, g, pg, kappa, \
+                       ccoef, nrhs)
     if (donly) {
         if (dag)
             LMX_PAD(1, MODE_DONLY);
@@ -212,6 +221,18 @@ int launch_wilson_padded(lmx_context* ctx, const double2* U, const double2* clov
 #undef LMX_PAD
     LMX_HIP(hipGetLastError());
     return LMX_OK;
+}
+
+int launch_wilson_padded(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
+                         double2* C, const double2* psi, int nw, int nrhs, int flags) {
+    return launch_padded<double2>(ctx, LinkF64{U, (int)ctx->V}, F64Clover{clover}, clover != nullptr, kappa, csw, C,
+                                  psi, nw, nrhs, flags);
+}
+
+int launch_wilson_padded_f32(lmx_context* ctx, const float2* U, const float2* clover, double kappa, double csw,
+                             float2* C, const float2* psi, int nw, int nrhs, int flags) {
+    return launch_padded<float2>(ctx, LinkF32{U, (int)ctx->V}, F32Clover{clover}, clover != nullptr, kappa, csw, C,
+                                 psi, nw, nrhs, flags);
 }
 
 }  // namespace lmx

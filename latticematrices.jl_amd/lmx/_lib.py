@@ -92,6 +92,7 @@ SIGNATURES = {
     "lmx_wilson_apply": (I, [P, P, P, I, I]),
     "lmx_wilson_apply_f32": (I, [P, P, P, I, I]),
     "lmx_wilson_apply_padded": (I, [P, P, P, I, I, I]),
+    "lmx_wilson_apply_padded_f32": (I, [P, P, P, I, I, I]),
     "lmx_ddagd_apply": (I, [P, P, P, P, I]),
     "lmx_cg_params_default": (None, [ctypes.POINTER(CGParams)]),
     "lmx_cg_ddagd": (I, [P, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),

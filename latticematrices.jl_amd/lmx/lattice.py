@@ -552,24 +552,27 @@ def mul_f32_(C, A, psi):
 
 def mul_padded_(C, A, psi, nw: int):
     """mul!(C, A, psi) on arrays in the reference's own storage (no
-    conversion): C and psi are device complex128 tensors laid out as
-    LatticeMatrix.A, i.e. C-order [nrhs?, Lt+2nw, Lz+2nw, Ly+2nw, Lx+2nw, 4, 3]
-    (column-major [3,4,Lx+2nw,...]).  psi's halo is ignored (in-kernel wrap
-    with the lattice's phases); only C's core sites are written.  A is a
-    Wilson(-clover) operator, its adjoint, or the hopping-only operator."""
+    conversion): C and psi are device complex128 (ComplexF64) or complex64
+    (ComplexF32) tensors laid out as LatticeMatrix.A, i.e. C-order [nrhs?,
+    Lt+2nw, Lz+2nw, Ly+2nw, Lx+2nw, 4, 3] (column-major [3,4,Lx+2nw,...]).
+    psi's halo is ignored (in-kernel wrap with the lattice's phases); only C's
+    core sites are written.  A is a Wilson(-clover) operator, its adjoint, or
+    the hopping-only operator; complex64 fields use FP32 links and arithmetic."""
     h, flags = _wilson_handle(A)
     lat = (A.parent if isinstance(A, Adjoint) else A).lattice
     per = int(np.prod([d + 2 * nw for d in lat.dims])) * 12
+    dt = psi.dtype if isinstance(psi, torch.Tensor) else None
     for t in (C, psi):
-        if not (isinstance(t, torch.Tensor) and t.dtype == torch.complex128 and t.is_contiguous()
-                and t.device == lat.device):
-            raise ArgumentError(_lib.LMX_ERR_ARG, "padded fields must be contiguous complex128 device tensors")
+        if not (isinstance(t, torch.Tensor) and t.dtype in (torch.complex128, torch.complex64) and t.dtype == dt
+                and t.is_contiguous() and t.device == lat.device):
+            raise ArgumentError(_lib.LMX_ERR_ARG, "padded fields must be contiguous device tensors, both complex128 "
+                                                  "or both complex64")
         if t.numel() % per != 0:
             raise ArgumentError(_lib.LMX_ERR_ARG, f"array of {t.numel()} elements is not a multiple of {per}")
     if C.numel() != psi.numel():
         raise ArgumentError(_lib.LMX_ERR_ARG, "C and psi must hold the same number of fields")
-    check(lib().lmx_wilson_apply_padded(h, ctypes.c_void_p(_ptr(C)), ctypes.c_void_p(_ptr(psi)), int(nw), flags,
-                                        psi.numel() // per))
+    fn = lib().lmx_wilson_apply_padded if dt == torch.complex128 else lib().lmx_wilson_apply_padded_f32
+    check(fn(h, ctypes.c_void_p(_ptr(C)), ctypes.c_void_p(_ptr(psi)), int(nw), flags, psi.numel() // per))
     return C
 
 

@@ -494,3 +494,49 @@ def test_single_precision_apply_many_rhs(gpu):
         y1 = lmx.spinor_f32(lat)
         lmx.mul_f32_(y1, lmx.adjoint(D), x[r:r + 1].contiguous())
         assert torch.equal(y[r:r + 1], y1)
+
+
+@pytest.mark.parametrize("nw,phases,csw", [(1, (1, 1, 1, -1), 0.0), (2, (np.exp(0.4j), -1, 1j, -1), 0.0),
+                                           (1, (1, 1, 1, -1), 1.2)])
+def test_apply_on_single_precision_reference_storage(gpu, nw, phases, csw):
+    # the same on LatticeMatrix{ComplexF32}.A arrays: FP32 arithmetic within the
+    # reference's 2e-5 (test/wilson_halfspin.jl:105), halo never read or written
+    L = (4, 6, 2, 8)
+    print(f"padded FP32 apply nw={nw} phases={phases} csw={csw}", flush=True)
+    U = gen.links(L, "haar", 83)
+    rng = np.random.default_rng(84)
+    P = [d + 2 * nw for d in L]
+    nrhs = 2
+    shape = (nrhs, P[3], P[2], P[1], P[0], 4, 3)
+    pad = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    core = (slice(None),) + tuple(slice(nw, nw + d) for d in reversed(L)) + (slice(None), slice(None))
+    psi_core = np.ascontiguousarray(pad[core]).reshape(nrhs, -1, 4, 3).astype(np.complex128)
+    lat = lmx.Lattice(L, phases)
+    Ug = lat.gauge_from_numpy(U)
+    D = lmx.WilsonDiracOperator4D(Ug, 0.118, csw)
+    F = orc.clover_field_strength(U, L) if csw else None
+    psi_d = torch.from_numpy(pad.copy()).to(lat.device)
+    sentinel = np.full(shape, 7.0 - 3.0j, dtype=np.complex64)
+    worst = 0.0
+    for flags in ((0, 1, 2, 3) if csw == 0 else (0, 1)):
+        A = lmx.WilsonDiracOperator4D_Donly(Ug) if flags & 2 else D
+        if flags & 1:
+            A = lmx.adjoint(A)
+        C_d = torch.from_numpy(sentinel.copy()).to(lat.device)
+        lmx.mul_padded_(C_d, A, psi_d, nw)
+        C = C_d.cpu().numpy()
+        for i in range(nrhs):
+            if csw:
+                ref = orc.wilson_clover(U, F, psi_core[i], L, 0.118, csw, phases=phases, dagger=bool(flags & 1))
+            else:
+                ref = orc.wilson(U, psi_core[i], L, 0.118, phases, dagger=bool(flags & 1), donly=bool(flags & 2))
+            got = np.ascontiguousarray(C[i][core[1:]]).reshape(-1, 4, 3)
+            e = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+            worst = max(worst, e)
+            assert e <= 2e-5, (flags, i, e)
+        halo = np.ones(shape, dtype=bool)
+        halo[core] = False
+        assert np.all(C[halo] == sentinel[halo])
+    print(f"  worst relative error vs FP64 oracle {worst:.2e}", flush=True)
+    with pytest.raises(lmx.ArgumentError):  # mixed element types
+        lmx.mul_padded_(torch.zeros(shape, dtype=torch.complex128, device=lat.device), D, psi_d, nw)
