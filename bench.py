@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--nrhs", type=int, default=12)
     ap.add_argument("--precision", type=str, default="h16", choices=["h16", "f16", "f32"])
+    ap.add_argument("--alt-precision", type=str, default="f32,f16",
+                    help="comma-separated inner precisions (f32, f16, h16) timed once each after the headline solve "
+                         "(reported, not the value); 'none' for none")
     ap.add_argument("--rhs-block", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--dslash-reps", type=int, default=50)
@@ -233,6 +236,33 @@ def main():
     log(f"bench: mixed {args.precision} x{n}: {tm:.3f} s/batch = {per_rhs:.4f} s/RHS; speedup vs FP64 "
         f"{t64 / per_rhs:.2f}x; max true rel residual {max(true_res):.3e}; RHS0 error vs FP64 {err0:.3e}")
 
+    # the configuration's own inner format (FP32 spinors and links) and FP16
+    # links with FP32 spinors, each timed once after the headline solve
+    alt = []
+    for prec in [q for q in args.alt_precision.split(",") if q and q != "none" and q != args.precision]:
+        if prec not in ("f32", "f16", "h16"):
+            raise SystemExit(f"unknown precision {prec}")
+        lmx.autotune(D, prec, n, None)
+
+        def solve_alt():
+            lmx.clear_matrix_(xm)
+            return lmx.cg_batched_(xm, D, bg, precision=prec, rtol=args.rtol, use_graph=bool(args.graph))
+
+        solve_alt()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        resa = solve_alt()
+        torch.cuda.synchronize()
+        ta = (time.perf_counter() - t0) / n
+        erra = float(np.linalg.norm(xm.to_numpy()[0] - x64) / np.linalg.norm(x64))
+        a = {"precision": prec, "s_per_rhs": ta, "speedup_vs_fp64": t64 / ta,
+             "iterations": [q.iterations for q in resa], "reliable_updates": [q.reliable_updates for q in resa],
+             "true_relative_residual_max": max(q.true_relative_residual for q in resa), "rhs0_error_vs_fp64": erra}
+        alt.append(a)
+        log(f"bench: mixed {prec} x{n} (one timed solve): {ta:.4f} s/RHS; speedup vs FP64 {t64 / ta:.2f}x; "
+            f"iterations {min(a['iterations'])}-{max(a['iterations'])}; max true rel residual "
+            f"{a['true_relative_residual_max']:.3e}; RHS0 error vs FP64 {erra:.3e}")
+
     traffic, traffic_src = measured_traffic(L)
     cpu = None
     if not args.no_cpu:
@@ -277,6 +307,7 @@ def main():
                                "(FP64 D_W, 32^4)",
                      "bytes_per_launch": bytes_d64, "us_per_launch": t_d64 * 1e3,
                      "traffic_source": traffic_src},
+        "alt_precision": alt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
