@@ -52,8 +52,8 @@ struct InnerVec {
 struct MixPtrs {
     float4* xlo;  // inner solution: FP32 component pairs, float4 [nrhs][6][V]
     InnerVec r, p[kXBatch], t;  // p ring: iteration k reads p[k % kXBatch], writes p[(k + 1) % kXBatch]
-    double2* r64;
-    double2* t64;
+    double2* r64;  // A x of a reliable update (FP64); the FP64 residual is formed on the fly from it
+    double2* t64;  // D x, the intermediate of that A x
     double* partials;
     MixRhs* rhs;
     MixBatch* batch;
@@ -199,10 +199,11 @@ __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ parti
     }
 }
 
-// r64 = b - A x (A x already in r64); r_lo = p = r64 (rounded to the inner
-// format); x_lo = 0; |r|^2 and |b|^2 per RHS.
+// r = b - A x with A x in ax64 (FP64; r itself is not kept in FP64: the next
+// reliable update recomputes it); r_lo = p = r rounded to the inner format;
+// x_lo = 0; |r|^2 and |b|^2 per RHS.
 template <class SO>
-__global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, double2* __restrict__ r64, SO rlo,
+__global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, const double2* __restrict__ ax64, SO rlo,
                                                  SO p, float4* __restrict__ xlo, int V, double* partials) {
     const int i = blockIdx.y;
     const int s = blockIdx.x * 256 + threadIdx.x;
@@ -213,9 +214,8 @@ __global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, 
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
             const size_t e = o + (size_t)k * V + s;
-            double2 bv = b[e], av = r64[e];
+            double2 bv = b[e], av = ax64[e];
             double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
-            r64[e] = rv;
             f[k] = mk<float>((float)rv.x, (float)rv.y);
             rr += rv.x * rv.x + rv.y * rv.y;
             bb += bv.x * bv.x + bv.y * bv.y;
@@ -300,9 +300,9 @@ __global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float
         accum_site(x, xlo, ring, V, rhs[i], mask, o, o4, i, s);
 }
 
-// reliable update, part 2: r64 = b - A x (A x in r64), r_lo = r64, |r|^2 in FP64
+// reliable update, part 2: r = b - A x (A x in ax64), r_lo = r, |r|^2 in FP64
 template <class SO>
-__global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__ b, double2* __restrict__ r64, SO rlo,
+__global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__ b, const double2* __restrict__ ax64, SO rlo,
                                                      int V, double* partials, const MixRhs* rhs,
                                                      const MixBatch* batch, int force) {
     const int i = blockIdx.y;
@@ -320,9 +320,8 @@ __global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__
 #pragma unroll
             for (int k = 0; k < 12; ++k) {
                 const size_t e = o + (size_t)k * V + s;
-                double2 bv = b[e], av = r64[e];
+                double2 bv = b[e], av = ax64[e];
                 double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
-                r64[e] = rv;
                 f[k] = mk<float>((float)rv.x, (float)rv.y);
                 v += rv.x * rv.x + rv.y * rv.y;
             }
