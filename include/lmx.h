@@ -198,8 +198,10 @@ int lmx_wilson_apply(lmx_wilson* op, void* out, const void* in, int flags, int n
 /* out = D in in single precision: mul!(C, D, psi) on ComplexF32 fields
  * (LatticeMatrix{...,ComplexF32}, test/wilson_halfspin.jl:105,124-130).  out
  * and in are float2 [nrhs][12][V] (component spin*3+colour); the operator's
- * links and clover cache are used rounded to FP32 and the arithmetic is
- * FP32.  Flags as lmx_wilson_apply (LMX_DAGGER, LMX_DONLY). */
+ * links and clover cache are used rounded to FP32 (copies the operator
+ * builds, with a device allocation, on first use and after
+ * lmx_wilson_update_links) and the arithmetic is FP32.  Flags as
+ * lmx_wilson_apply (LMX_DAGGER, LMX_DONLY). */
 int lmx_wilson_apply_f32(lmx_wilson* op, void* out, const void* in, int flags, int nrhs);
 /* The same mul! directly on the reference's storage, without conversion:
  * psi and C are LatticeMatrix.A arrays (complex128, column-major
