@@ -291,10 +291,13 @@ typedef struct {
     double ms_per_apply; /* measured D apply time of the chosen shape, per RHS */
 } lmx_tune_result;
 
-/* Time the D^dagger D inner dslash of `precision` over RHS blocks
- * {1,2,4,6,12} (those dividing nrhs first) and workgroup sizes, cache the
- * winner per (volume, precision, nrhs, arch) in cache_path (may be NULL) and
- * use it for later lmx_cg_batched calls with rhs_block = 0. */
+/* Time one group of the mixed solver's inner D / D^dagger iterations of
+ * `precision` over the launch shapes (RHS per launch {1,2,4,6,12}, RHS per
+ * work item {1,2,4}, t-slices per workgroup, z-ranges), cache the winner per
+ * (volume, precision, nrhs, clover term or not, arch) in cache_path (may be
+ * NULL: always measure; a cached shape this build cannot launch on this
+ * lattice is ignored and retuned) and use it for later lmx_cg_batched calls
+ * with rhs_block = 0. */
 int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char* cache_path, lmx_tune_result* out);
 
 /* ======================================================================= */

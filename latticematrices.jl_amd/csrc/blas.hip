@@ -1,9 +1,10 @@
 // blas.hip -- BLAS-1 and the fused FP64 CG vector kernels.
 //
-// Each reduction is one pass: every block reduces its 256 sites with the
-// fixed tree of lmx_device.h, publishes a partial, and the last block to
-// finish sums the partials in a fixed order and updates the device-resident
-// CG state (alpha, beta, convergence) without a host round trip.  The
+// Each reduction is two launches: every block of the producing kernel reduces
+// its 256 sites with the fixed tree of lmx_device.h and stores one partial;
+// a one-block k_finalize launch then sums the partials in a fixed order and
+// updates the device-resident CG state (alpha, beta, convergence) without a
+// host round trip.  The
 // element-wise updates use the reference's unfused forms
 //   axpby!(a, X, b, Y): Y = a*X + b*Y   src/LinearAlgebras/linearalgebra_D.jl:15-23
 // and the per-site dot kernel_dot_D (:3331-3341).

@@ -90,6 +90,7 @@ struct lmx_wilson {
 namespace lmx {
 
 constexpr int kCounters = 64;
+constexpr int kMaxTL = 32;  // t-slices per streaming workgroup at most (LDS of the fused reductions)
 constexpr int kMaxStates = 64;
 
 inline int nblocks(int64_t V) { return int((V + kRedBlock - 1) / kRedBlock); }

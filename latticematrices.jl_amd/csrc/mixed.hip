@@ -32,14 +32,17 @@ struct MixRhs {
     double alpha_q[kXBatch - 1];  // alphas of the deferred x_inner += alpha p, by ring slot
     int done, converged, reason, since, iterations, updates;
     int pmask;  // ring slots whose alpha p awaits the group's last iteration (or a flush)
-    int pad;
+    // retired (maximum iterations, breakdown, nonpositive curvature) with its
+    // inner solution and deferred alpha p not yet in x: the solve's final
+    // pass folds them in and computes the true residual
+    int fold;
 };
 struct MixBatch {
     int do_update;  // reliable update requested this iteration
     int n_active;   // RHS still iterating
     double delta;
     int rmax;
-    int pad;
+    int maxiter;    // an RHS stops once its inner iteration count reaches it
 };
 
 // An inner (low-precision) vector of nrhs fields: FP32 (n == null) or
@@ -112,18 +115,23 @@ __device__ __forceinline__ void row_partials(double a, double b, double* partial
     }
 }
 
-__device__ __forceinline__ void retire(MixRhs* st, MixBatch* batch, int reason, int converged) {
+__device__ __forceinline__ void retire(MixRhs* st, MixBatch* batch, int reason, int converged, int fold) {
     st->done = 1;
     st->reason = reason;
     st->converged = converged;
+    st->fold = fold;
     atomicSub(&batch->n_active, 1);
 }
 
 enum { MX_INIT = 0, MX_PAP = 1, MX_RR = 2, MX_TRUE = 3 };
 
-// flag: MX_TRUE -- run even without a reliable-update request (final);
+// flag: MX_TRUE -- run even without a reliable-update request (the final
+//                  pass: RHS still iterating, and retired RHS awaiting their fold);
 //       MX_RR   -- the iteration's phase j in its group: j < kXBatch - 1
 //                  deferred its x_inner += alpha p into ring slot j.
+// An RHS stops with LMX_CG_MAXITER when its iteration count reaches maxiter
+// (cg!, src/LinearAlgebras/cg.jl:105,135-136), in the iteration's own
+// finalize, so a graph replay that runs past maxiter leaves it untouched.
 template <int KIND>
 __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ partials, int nb, MixRhs* rhs,
                                                 MixBatch* batch, double rtol, double atol, int flag) {
@@ -131,7 +139,8 @@ __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ parti
     MixRhs* st = rhs + i;
     if (KIND == MX_PAP && i == 0 && threadIdx.x == 0) batch->do_update = 0;
     if (KIND == MX_TRUE && !flag && !batch->do_update) return;
-    if (st->done) return;
+    const bool final_fold = KIND == MX_TRUE && flag && st->done && st->fold;
+    if (st->done && !final_fold) return;
     if (threadIdx.x == 0) {
         if (KIND == MX_RR) {
             if (flag < kXBatch - 1) {
@@ -159,22 +168,24 @@ __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ parti
         st->updates = 0;
         st->beta = 0.0;
         if (!isfinite(A) || !isfinite(B))
-            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+            retire(st, batch, LMX_CG_BREAKDOWN, 0, 0);
         else if (st->rtrue <= st->tol)
-            retire(st, batch, LMX_CG_CONVERGED, 1);
+            retire(st, batch, LMX_CG_CONVERGED, 1, 0);
+        else if (batch->maxiter <= 0)
+            retire(st, batch, LMX_CG_MAXITER, 0, 0);
     } else if (KIND == MX_PAP) {  // A = p.Ap = |D p|^2
         st->pAp = A;
         if (!isfinite(A))
-            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+            retire(st, batch, LMX_CG_BREAKDOWN, 0, 1);
         else if (A <= 0.0)
-            retire(st, batch, LMX_CG_NONPOSITIVE_CURVATURE, 0);
+            retire(st, batch, LMX_CG_NONPOSITIVE_CURVATURE, 0, 1);
         else
             st->alpha = st->rr / A;
     } else if (KIND == MX_RR) {  // A = |r_inner|^2
         st->iterations += 1;
         st->since += 1;
         if (!isfinite(A)) {
-            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+            retire(st, batch, LMX_CG_BREAKDOWN, 0, 1);
             return;
         }
         const double rn = sqrt(A);
@@ -182,14 +193,28 @@ __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ parti
         st->rr_prev = st->rr;
         st->beta = A / st->rr;
         st->rr = A;
+        if (st->iterations >= batch->maxiter) {
+            // the final pass folds x_inner in; a true residual within the
+            // tolerance there still counts as converged
+            retire(st, batch, LMX_CG_MAXITER, 0, 1);
+            return;
+        }
         if (rn < batch->delta * st->rref || st->since >= batch->rmax || rn <= st->tol) atomicOr(&batch->do_update, 1);
+    } else if (final_fold) {  // MX_TRUE on a retired RHS: A = |b - A x|^2 after its fold
+        st->rtrue = sqrt(A);
+        st->updates += 1;
+        st->fold = 0;
+        if (st->reason == LMX_CG_MAXITER && isfinite(A) && st->rtrue <= st->tol) {
+            st->reason = LMX_CG_CONVERGED;
+            st->converged = 1;
+        }
     } else {  // MX_TRUE: A = |b - A x|^2 in FP64
         st->rtrue = sqrt(A);
         st->updates += 1;
         if (!isfinite(A)) {
-            retire(st, batch, LMX_CG_BREAKDOWN, 0);
+            retire(st, batch, LMX_CG_BREAKDOWN, 0, 0);
         } else if (st->rtrue <= st->tol) {
-            retire(st, batch, LMX_CG_CONVERGED, 1);
+            retire(st, batch, LMX_CG_CONVERGED, 1, 0);
         } else {
             st->beta = A / st->rr_prev;
             st->rr = A;
@@ -228,7 +253,7 @@ __global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, 
     row_partials(rr, bb, partials, gridDim.x, blockIdx.x);
 }
 
-__global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta, int rmax) {
+__global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta, int rmax, int maxiter) {
     const int t = threadIdx.x;
     if (t < nrhs) {
         MixRhs z;
@@ -241,6 +266,7 @@ __global__ void k_mx_reset(MixRhs* rhs, MixBatch* batch, int nrhs, double delta,
         batch->n_active = nrhs;
         batch->delta = delta;
         batch->rmax = rmax;
+        batch->maxiter = maxiter;
     }
 }
 
@@ -287,13 +313,20 @@ __device__ __forceinline__ void accum_site(double2* __restrict__ x, float4* __re
     }
 }
 
+// A reliable-update kernel works on RHS i when the iteration requested an
+// update and i still iterates, or, in the final (forced) pass, when i still
+// iterates or retired with its fold pending.
+__device__ __forceinline__ bool update_runs(const MixRhs& st, const MixBatch* batch, int force) {
+    return force ? (!st.done || st.fold) : (batch->do_update && !st.done);
+}
+
 // (grid-stride over the sites: a small grid, so that the launch costs little
 // on the iterations where the update is gated off)
 template <class SI>
 __global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float4* __restrict__ xlo, RingSlots<SI> ring,
                                                   int V, const MixRhs* rhs, const MixBatch* batch, int force) {
     const int i = blockIdx.y;
-    if ((!force && !batch->do_update) || rhs[i].done) return;
+    if (!update_runs(rhs[i], batch, force)) return;
     const size_t o = (size_t)i * 12 * V, o4 = (size_t)i * 6 * V;
     const int mask = rhs[i].pmask;  // slots not yet overwritten by a later p update
     for (int s = blockIdx.x * 256 + threadIdx.x; s < V; s += gridDim.x * 256)
@@ -306,7 +339,7 @@ __global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__
                                                      int V, double* partials, const MixRhs* rhs,
                                                      const MixBatch* batch, int force) {
     const int i = blockIdx.y;
-    if ((!force && !batch->do_update) || rhs[i].done) return;
+    if (!update_runs(rhs[i], batch, force)) return;
     static_assert(kRedBlock == 256, "one partial per 256-site block");
     const int nb = (V + 255) >> 8;
     // grid-stride over the 256-site blocks (a small grid, cheap when gated
@@ -573,7 +606,7 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     dim3 grid(nblocks(c->V), nrhs), blk(256);
     const int rmax = params->reliable_maxiter > 0 ? params->reliable_maxiter : 100;
     hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(64), 0, c->stream, R.w.rhs, R.w.batch, nrhs, params->reliable_delta,
-                       rmax);
+                       rmax, params->maxiter);
     LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
     if (R.spinor_prec == LMX_H16)
         enqueue_init<SpH16Out>(R, grid);
@@ -631,8 +664,9 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     LMX_TRY(rc);
-    // RHS stopped by maxiter: fold the inner solution in and report the true residual
-    // the ring slots marked in pmask hold the still deferred alpha p
+    // RHS stopped by maxiter (or retired by a breakdown): fold the inner
+    // solution in and report the true residual; the ring slots marked in
+    // pmask hold the still deferred alpha p
     enqueue_accum(R, grid, 1);
     LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
     if (R.spinor_prec == LMX_H16)
@@ -665,8 +699,8 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
 // from their own buffers) for every RHS
 // blocking (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,4})
 // and streaming depth (t-slices per workgroup: automatic, 4, 8) and keep
-// the fastest per (volume, precision, nrhs, arch); optionally persisted as
-// text lines "V precision nrhs arch rhs_block us_per_rhs".
+// the fastest per (volume, precision, nrhs, clover term or not, arch);
+// optionally persisted as text lines (cache_lookup).
 // ---------------------------------------------------------------------------
 static std::string arch_name(int dev) {
     hipDeviceProp_t prop;
@@ -676,22 +710,47 @@ static std::string arch_name(int dev) {
     return c == std::string::npos ? a : a.substr(0, c);
 }
 
-static bool cache_lookup(const char* path, long V, int prec, int nrhs, const std::string& arch, int* block,
-                         double* us) {
+// A cached launch shape is used only if it is one this build can launch on
+// this lattice: RHS per work item 1, 2 or 4 dividing the RHS per launch,
+// t-slices per workgroup dividing Lt (at most kMaxTL), z-ranges 1, 2 or 4
+// dividing the 256-site blocks of a t-slice, streaming fields only where the
+// lattice streams.  Anything else (a foreign or damaged line) is retuned.
+static bool block_launchable(const lmx_context* c, int blk) {
+    if (blk <= 0 || (blk >> 24) != 0) return false;
+    const int nr = blk & 15, per = (blk >> 4) & 0xff, tl = (blk >> 12) & 0xff, zs = (blk >> 20) & 0xf;
+    if (nr != 1 && nr != 2 && nr != 4) return false;
+    if (per < nr || per % nr != 0) return false;
+    const long V3 = (long)c->g.L[0] * c->g.L[1] * c->g.L[2];
+    const bool streams = V3 % 256 == 0;
+    if (tl != 0 && (!streams || tl > kMaxTL || c->g.L[3] % tl != 0)) return false;
+    if (zs != 0 && (!streams || (zs != 1 && zs != 2 && zs != 4) || (V3 / 256) % zs != 0)) return false;
+    return true;
+}
+
+// Cache lines: "lmx1 V precision nrhs clover arch rhs_block us_per_rhs"; the
+// key is (V, precision, nrhs, clover or not, arch), the last matching
+// launchable line wins.
+static constexpr const char* kTuneTag = "lmx1";
+static bool cache_lookup(const char* path, const lmx_context* c, int prec, int nrhs, int clover,
+                         const std::string& arch, int* block, double* us) {
     if (!path) return false;
     FILE* f = fopen(path, "r");
     if (!f) return false;
-    char a[64];
-    long v;
-    int p, n, blk;
-    double t;
+    char line[256];
     bool found = false;
-    while (fscanf(f, "%ld %d %d %63s %d %lf", &v, &p, &n, a, &blk, &t) == 6)
-        if (v == V && p == prec && n == nrhs && arch == a) {
-            *block = blk;
-            *us = t;
-            found = true;
-        }
+    while (fgets(line, sizeof(line), f)) {
+        char tag[16], a[64];
+        long v;
+        int p, n, cl, blk;
+        double t;
+        if (sscanf(line, "%15s %ld %d %d %d %63s %d %lf", tag, &v, &p, &n, &cl, a, &blk, &t) != 8) continue;
+        if (strcmp(tag, kTuneTag) != 0 || v != (long)c->V || p != prec || n != nrhs || cl != clover || arch != a)
+            continue;
+        if (!block_launchable(c, blk) || !(t > 0.0)) continue;
+        *block = blk;
+        *us = t;
+        found = true;
+    }
     fclose(f);
     return found;
 }
@@ -709,7 +768,8 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     const std::string arch = arch_name(c->device);
     int best = 0;
     double best_us = 0.0;
-    if (cache_lookup(cache_path, (long)c->V, precision, nrhs, arch, &best, &best_us)) {
+    const int clover = op->csw != 0.0 ? 1 : 0;
+    if (cache_lookup(cache_path, c, precision, nrhs, clover, arch, &best, &best_us)) {
         op->tuned_block[precision][nrhs] = best;
         out->rhs_block = best;
         out->block_size = 256;
@@ -839,7 +899,8 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     if (cache_path) {
         FILE* f = fopen(cache_path, "a");
         if (f) {
-            fprintf(f, "%ld %d %d %s %d %.3f\n", (long)c->V, precision, nrhs, arch.c_str(), best, best_us);
+            fprintf(f, "%s %ld %d %d %d %s %d %.3f\n", kTuneTag, (long)c->V, precision, nrhs, clover, arch.c_str(), best,
+                    best_us);
             fclose(f);
         }
     }

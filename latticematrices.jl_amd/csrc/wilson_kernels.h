@@ -170,7 +170,6 @@ struct EpiArgs {
 };
 
 enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
-constexpr int kMaxTL = 32;  // t-slices per workgroup at most (LDS of the fused reduction)
 #ifndef LMX_CARRY_NEXT
 #define LMX_CARRY_NEXT 1
 #endif

@@ -448,6 +448,15 @@ class DdagDOp:
             raise TypeError("DdagDOp(D) without a temporary field needs a DiracOp")
 
 
+def _wilson_object(A):
+    """The WilsonDiracOperator4D behind a Wilson-family operator or its adjoint."""
+    if isinstance(A, Adjoint):
+        return _wilson_object(A.parent)
+    if isinstance(A, WilsonDiracOperator4D_Donly):
+        return A._w
+    return A if isinstance(A, WilsonDiracOperator4D) else None
+
+
 def _wilson_handle(A):
     if isinstance(A, WilsonDiracOperator4D):
         return A._op, 0
@@ -529,6 +538,18 @@ def spinor_f32_to_numpy(t):
     return np.ascontiguousarray(c.transpose(0, 2, 1)).reshape(c.shape[0], c.shape[2], 4, 3)
 
 
+def _refresh_low_precision(op):
+    """The single-precision and mixed paths read the operator's rounded link
+    (and clover) copies, built from the links when the operator was created
+    or last updated; the FP64 path reads the links themselves.  Rebuild the
+    copies when the links object or its version (GaugeField.core_epoch: an
+    in-place torch write, a replaced .data, mark_halo_dirty_) changed since,
+    so that every precision applies the same operator."""
+    src, epoch = op._source
+    if src is not op.U or epoch != op.U.core_epoch:
+        op.update_links()
+
+
 def mul_f32_(C, A, psi):
     """mul!(C, A, psi) on single-precision fields (ComplexF32,
     test/wilson_halfspin.jl:105,124-130): C and psi are float32 device tensors
@@ -546,6 +567,7 @@ def mul_f32_(C, A, psi):
     lat = op.lattice
     if C.shape[2] != lat.V or C.device != lat.device:
         raise ArgumentError(_lib.LMX_ERR_ARG, "fields do not match the operator's lattice")
+    _refresh_low_precision(_wilson_object(A))
     check(lib().lmx_wilson_apply_f32(h, ctypes.c_void_p(_ptr(C)), ctypes.c_void_p(_ptr(psi)), flags, C.shape[0]))
     return C
 
@@ -571,6 +593,8 @@ def mul_padded_(C, A, psi, nw: int):
             raise ArgumentError(_lib.LMX_ERR_ARG, f"array of {t.numel()} elements is not a multiple of {per}")
     if C.numel() != psi.numel():
         raise ArgumentError(_lib.LMX_ERR_ARG, "C and psi must hold the same number of fields")
+    if dt == torch.complex64:
+        _refresh_low_precision(_wilson_object(A))
     fn = lib().lmx_wilson_apply_padded if dt == torch.complex128 else lib().lmx_wilson_apply_padded_f32
     check(fn(h, ctypes.c_void_p(_ptr(C)), ctypes.c_void_p(_ptr(psi)), int(nw), flags, psi.numel() // per))
     return C
@@ -791,6 +815,8 @@ def cg_batched_(x: SpinorField, D: WilsonDiracOperator4D, b: SpinorField, precis
                                  precision=_PREC[precision], reliable_delta=float(reliable_delta),
                                  reliable_maxiter=int(reliable_maxiter), check_every=int(check_every),
                                  use_graph=int(bool(use_graph)), rhs_block=int(rhs_block))
+    if precision != "f64":
+        _refresh_low_precision(D)
     nbytes = ctypes.c_size_t()
     check(lib().lmx_cg_batched_workspace(D._op, x.nrhs, ctypes.byref(params), ctypes.byref(nbytes)))
     if workspace is None or workspace.numel() < nbytes.value:
@@ -808,6 +834,7 @@ def cg_batched_(x: SpinorField, D: WilsonDiracOperator4D, b: SpinorField, precis
 
 def autotune(D: WilsonDiracOperator4D, precision: str, nrhs: int, cache_path: str | None = None):
     """Pick the RHS blocking of the mixed solver's dslash (lmx_autotune)."""
+    _refresh_low_precision(D)
     out = _lib.TuneResult()
     check(lib().lmx_autotune(D._op, _PREC[precision], int(nrhs),
                              cache_path.encode() if cache_path else None, ctypes.byref(out)))

@@ -225,3 +225,57 @@ def test_z_range_shapes_are_bit_identical(gpu):
             ref = got
         else:
             assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), zs
+
+
+@pytest.mark.parametrize("check_every", [1, 10])
+@pytest.mark.parametrize("maxiter", [5, 6, 7, 13])
+def test_graph_stops_at_maxiter(gpu, maxiter, check_every):
+    # cg! stops after exactly maxiter iterations with reason maximum_iterations
+    # (src/LinearAlgebras/cg.jl:105,135-136); graph replays run whole chunks,
+    # so the device retires each RHS in the iteration that reaches maxiter and
+    # the replayed remainder of the chunk leaves it untouched: same iterate as
+    # the eager launches, its inner solution folded in, its true residual
+    # reported
+    L, kappa = (8, 4, 4, 8), 0.12
+    print(f"graph maxiter={maxiter} check_every={check_every}", flush=True)
+    U, b, lat, Ug, D = setup(L, "weak", 385, kappa, nrhs=2)
+    bg = lat.spinor_from_numpy(b)
+    A = orc.NormalOp(orc.OP_DDAGD_WILSON, L, U=U, kappa=kappa)
+    x, xe = lat.spinor(2), lat.spinor(2)
+    res = lmx.cg_batched_(x, D, bg, precision="h16", maxiter=maxiter, check_every=check_every, use_graph=True)
+    rese = lmx.cg_batched_(xe, D, bg, precision="h16", maxiter=maxiter, check_every=check_every, use_graph=False)
+    xs = x.to_numpy()
+    for i, (r, re_) in enumerate(zip(res, rese)):
+        assert r.iterations == maxiter and re_.iterations == maxiter
+        assert r.reason == "maximum_iterations" and not r.converged
+        assert r.reliable_updates == re_.reliable_updates
+        tr = np.linalg.norm(A.apply(xs[i]) - b[i]) / np.linalg.norm(b[i])
+        assert abs(tr - r.true_relative_residual) <= 1e-6 * tr
+    assert np.array_equal(xs, xe.to_numpy())
+
+
+def test_low_precision_copies_follow_in_place_link_writes(gpu):
+    # the FP64 path reads the links themselves, the single-precision and mixed
+    # paths the operator's rounded copies: after an in-place write to the
+    # links every path must apply the new operator
+    L, kappa = (8, 4, 4, 8), 0.12
+    U, b, lat, Ug, D = setup(L, "weak", 395, kappa, nrhs=2)
+    bg = lat.spinor_from_numpy(b)
+    lmx.cg_batched_(lat.spinor(2), D, bg, precision="h16")  # builds the copies from the first links
+    psi32 = lmx.spinor_f32_from_numpy(lat, b)
+    out32 = lmx.spinor_f32(lat, 2)
+    lmx.mul_f32_(out32, D, psi32)
+    G2 = lat.gauge_from_numpy(gen.links(L, "weak", 396, eps=0.2))
+    Ug.data.copy_(G2.data)  # in place: same tensor, new version
+    lmx.mul_f32_(out32, D, psi32)
+    out64 = lat.spinor(2)
+    lmx.mul_(out64, D, bg)
+    got, ref = lmx.spinor_f32_to_numpy(out32), out64.to_numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+    D2 = lmx.WilsonDiracOperator4D(G2, kappa)
+    x1, x2 = lat.spinor(2), lat.spinor(2)
+    r1 = lmx.cg_batched_(x1, D, bg, precision="h16")
+    r2 = lmx.cg_batched_(x2, D2, bg, precision="h16")
+    assert [r.iterations for r in r1] == [r.iterations for r in r2]
+    assert np.array_equal(x1.to_numpy(), x2.to_numpy())
+    assert all(r.converged for r in r1)
