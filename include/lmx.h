@@ -236,6 +236,20 @@ void lmx_cg_params_default(lmx_cg_params* p);
 int lmx_cg_ddagd(lmx_wilson* op, void* x, const void* b, void* r, void* p, void* Ap, void* tmp,
                  const lmx_cg_params* params, lmx_cg_result* result);
 
+/* No counterpart in the reference: lmx_cg_ddagd for callers that cannot
+ * allocate device memory (code between graph captures, tight serving or HMC
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results (bit for bit) as lmx_cg_ddagd; it calls no hipMalloc / hipFree.
+ * The reduction partials and the CG state live in a static device array of
+ * 1 MiB + 256 B that each GPU holds until the process ends, so the lattice
+ * volume is at most 2^24 sites (64^4; LMX_ERR_ARG beyond).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (several host
+ * threads or streams at once) race and must be serialised by the caller.
+ * The operator must be a Wilson or Wilson-clover operator whose clover
+ * cache was built at creation or by lmx_wilson_update_links. */
+int lmx_cg_ddagd_noalloc(lmx_wilson* op, void* x, const void* b, void* r, void* p, void* Ap, void* tmp,
+                         const lmx_cg_params* params, lmx_cg_result* result);
+
 /* cg! with a caller operator A (callback on the context stream), FP64,
  * single field -- cg!(x, A, b, r, p, Ap) for any mul!-implementing A. */
 int lmx_cg_generic(lmx_context* ctx, lmx_apply_fn A, void* A_user, void* x, const void* b, void* r, void* p,

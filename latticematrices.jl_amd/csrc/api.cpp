@@ -97,6 +97,80 @@ int ensure_lo_links(lmx_wilson* op) {
     return LMX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free FP64 CG (lmx_cg_ddagd_noalloc).  Its reduction partials and
+// CG state live in `scratch`, a static device array: every GPU the library
+// runs on holds its own copy until the process ends, and every call on that
+// GPU shares it, so calls that overlap on one GPU race.  The host side never
+// uses the array's name as an address: lmx_context_create resolves the
+// device address once per device (with that device current) and keeps it.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocStateBytes = 256;                 // one CGState, padded
+constexpr int64_t kNoAllocMaxVolume = int64_t(1) << 24;    // 64^4 sites
+constexpr size_t kNoAllocScratchBytes =
+    kNoAllocStateBytes + sizeof(double) * 2 * (size_t)(kNoAllocMaxVolume / kRedBlock);  // 1 MiB + 256 B
+static_assert(sizeof(CGState) <= kNoAllocStateBytes, "CGState must fit its scratch slot");
+
+}  // namespace lmx
+
+static __device__ __align__(256) unsigned char scratch[lmx::kNoAllocScratchBytes];
+
+namespace lmx {
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[kMaxDevices];
+
+// Device address of this device's `scratch`; `device` must be current.
+static int noalloc_scratch_address(int device, void** out) {
+    LMX_CHECK(device >= 0 && device < kMaxDevices, LMX_ERR_UNSUPPORTED, "device ordinal %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device] == nullptr) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        LMX_CHECK(e == hipSuccess && ptr != nullptr, LMX_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s",
+                  device, hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return LMX_OK;
+}
+
+// cg!(x, DdagDOp(D, tmp), b, r, p, Ap) in FP64 on the context c, whose
+// partials and state buffers the kernels use as their scratch.
+static int cg_ddagd_run(lmx_wilson* op, lmx_context* c, void* x, const void* b, void* r, void* p, void* Ap,
+                        void* tmp, const lmx_cg_params* params, lmx_cg_result* result) {
+    CGState* st = c->state;
+    const double2* U = op->links;
+    const double2* F = op->clover64;
+    double2 *X = (double2*)x, *R = (double2*)r, *P = (double2*)p, *AP = (double2*)Ap, *T = (double2*)tmp;
+    const double2* B = (const double2*)b;
+    Epilogue gate{0, nullptr, st, nullptr, nullptr};
+    Epilogue pap{1, P, st, c->partials, nullptr};
+    LMX_TRY(launch_cg_init_state(c, st, params->rtol, params->atol, params->maxiter));
+    LMX_TRY(launch_dot(c, B, B, 1, st, nullptr));
+    LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, T, X, 1, 0, gate));
+    LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, AP, T, 1, LMX_DAGGER, gate));
+    LMX_TRY(launch_cg_setup_r(c, B, AP, R, P, st));
+    LMX_TRY(read_state(c, st, c->host_state));
+    const int chunk = chunk_of(params);
+    const long max_polls = (long)params->maxiter / chunk + 3;
+    long polls = 0;
+    while (!c->host_state->done) {
+        LMX_CHECK(++polls <= max_polls, LMX_ERR_HIP, "CG state did not advance (iteration %d)",
+                  c->host_state->iteration);
+        for (int k = 0; k < chunk; ++k) {
+            LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, T, P, 1, 0, gate));
+            LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, AP, T, 1, LMX_DAGGER, pap));
+            LMX_TRY(launch_cg_update_xr(c, X, R, P, AP, st));
+            LMX_TRY(launch_cg_update_p(c, R, P, st));
+        }
+        LMX_TRY(read_state(c, st, c->host_state));
+    }
+    fill_result(*c->host_state, result);
+    return LMX_OK;
+}
+
 static int refresh_derived(lmx_wilson* op) {
     lmx_context* ctx = op->ctx;
     if (op->csw != 0.0) {
@@ -178,6 +252,13 @@ int lmx_context_create(lmx_context** out, const int32_t dims[4], const double* p
         delete c;
         set_error("hipGetDevice failed: %s", hipGetErrorString(e));
         return LMX_ERR_HIP;
+    }
+    {
+        int rc = noalloc_scratch_address(c->device, &c->noalloc_scratch);
+        if (rc != LMX_OK) {
+            delete c;
+            return rc;
+        }
     }
     c->partials_cap = 2 * (size_t)nblocks(V) * 16 + 1024;
     if (hipMalloc(&c->partials, c->partials_cap * sizeof(double)) != hipSuccess ||
@@ -469,36 +550,31 @@ int lmx_cg_ddagd(lmx_wilson* op, void* x, const void* b, void* r, void* p, void*
     const char* names[6] = {"x", "b", "r", "p", "Ap", "tmp"};
     LMX_TRY(validate_cg(f, names, 6, params));
     LMX_CHECK(result != nullptr, LMX_ERR_ARG, "result is NULL");
-    lmx_context* c = op->ctx;
-    CGState* st = c->state;
-    const double2* U = op->links;
-    const double2* F = op->clover64;
-    double2 *X = (double2*)x, *R = (double2*)r, *P = (double2*)p, *AP = (double2*)Ap, *T = (double2*)tmp;
-    const double2* B = (const double2*)b;
-    Epilogue gate{0, nullptr, st, nullptr, nullptr};
-    Epilogue pap{1, P, st, c->partials, c->counters + 1};
-    LMX_TRY(launch_cg_init_state(c, st, params->rtol, params->atol, params->maxiter));
-    LMX_TRY(launch_dot(c, B, B, 1, st, nullptr));
-    LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, T, X, 1, 0, gate));
-    LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, AP, T, 1, LMX_DAGGER, gate));
-    LMX_TRY(launch_cg_setup_r(c, B, AP, R, P, st));
-    LMX_TRY(read_state(c, st, c->host_state));
-    const int chunk = chunk_of(params);
-    const long max_polls = (long)params->maxiter / chunk + 3;
-    long polls = 0;
-    while (!c->host_state->done) {
-        LMX_CHECK(++polls <= max_polls, LMX_ERR_HIP, "CG state did not advance (iteration %d)",
-                  c->host_state->iteration);
-        for (int k = 0; k < chunk; ++k) {
-            LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, T, P, 1, 0, gate));
-            LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, AP, T, 1, LMX_DAGGER, pap));
-            LMX_TRY(launch_cg_update_xr(c, X, R, P, AP, st));
-            LMX_TRY(launch_cg_update_p(c, R, P, st));
-        }
-        LMX_TRY(read_state(c, st, c->host_state));
-    }
-    fill_result(*c->host_state, result);
-    return LMX_OK;
+    return cg_ddagd_run(op, op->ctx, x, b, r, p, Ap, tmp, params, result);
+}
+
+int lmx_cg_ddagd_noalloc(lmx_wilson* op, void* x, const void* b, void* r, void* p, void* Ap, void* tmp,
+                         const lmx_cg_params* params, lmx_cg_result* result) {
+    clear_error();
+    LMX_TRY(check_op(op));
+    const void* f[6] = {x, b, r, p, Ap, tmp};
+    const char* names[6] = {"x", "b", "r", "p", "Ap", "tmp"};
+    LMX_TRY(validate_cg(f, names, 6, params));
+    LMX_CHECK(result != nullptr, LMX_ERR_ARG, "result is NULL");
+    const lmx_context* c = op->ctx;
+    LMX_CHECK(c->V <= kNoAllocMaxVolume, LMX_ERR_ARG,
+              "lattice volume %lld exceeds the allocation-free CG's scratch (at most %lld sites)", (long long)c->V,
+              (long long)kNoAllocMaxVolume);
+    LMX_CHECK(c->noalloc_scratch != nullptr, LMX_ERR_HIP, "no scratch address for device %d", c->device);
+    // the same drivers and kernels as lmx_cg_ddagd, with the context's
+    // partials and state replaced by this GPU's static scratch
+    lmx_context local = *c;
+    local.state = (CGState*)c->noalloc_scratch;
+    local.partials = (double*)((unsigned char*)c->noalloc_scratch + kNoAllocStateBytes);
+    local.partials_cap = (kNoAllocScratchBytes - kNoAllocStateBytes) / sizeof(double);
+    local.counters = nullptr;
+    local.dot_out = nullptr;
+    return cg_ddagd_run(op, &local, x, b, r, p, Ap, tmp, params, result);
 }
 
 int lmx_cg_generic(lmx_context* c, lmx_apply_fn A, void* A_user, void* x, const void* b, void* r, void* p, void* Ap,

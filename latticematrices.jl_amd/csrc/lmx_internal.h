@@ -68,6 +68,9 @@ struct lmx_context {
     // Stencil launch-shape overrides (0 = automatic): t-slices per workgroup
     // and 3-D sub-ranges, from LMX_STREAM_TL / LMX_STREAM_ZS at creation.
     int tune_tl, tune_zs;
+    // Device address of this GPU's static scratch array (the allocation-free
+    // CG's partials and state), resolved once per device at creation.
+    void* noalloc_scratch;
 };
 
 struct lmx_wilson {
