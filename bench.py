@@ -14,11 +14,14 @@ time-to-solution per RHS.  Links are generated on the host (numpy, seeded)
 and uploaded, sources on the device, all before the timed region; the timed
 region holds device work only (plus the solver's host convergence polls).
 
-Also measured: the FP64 D_W launch (roofline kernel, 960 B/site algorithmic:
-links 576 + psi 192 + out 192) timed with HIP events on the library stream,
-the mixed multi-RHS dslash (links/n + in + out per site per RHS: 144/n + 104
-B with 16-bit spinors and FP16 links), and a plain 1 GiB copy
-kernel used to calibrate the rocprof byte counters.
+Also measured, each with HIP events on the library stream: the mixed solve's
+two inner stencil kernels at the autotuned shape (`roofline`: D^dagger with
+the fused r / x update, the dominant kernel of the timed solve; and D with
+|Dp|^2), the FP64 D_W launch (960 B/site algorithmic: links 576 + psi 192 +
+out 192), and a plain 1 GiB copy kernel used to calibrate the rocprof byte
+counters.  The CPU baseline is one full solve of the same FP64 system by the
+reference's own CPU algorithm (oracle/: padded storage, halo refresh per
+operator application, unfused BLAS-1, OpenMP threads).
 
 Prints one JSON line.  Run: python bench.py [--steps K] [--warmup W]
 """
@@ -57,7 +60,7 @@ def parse():
     ap.add_argument("--rhs-block", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--dslash-reps", type=int, default=50)
-    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-maxiter", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--mode", type=str, default="full", choices=["full", "kernels"],
@@ -80,21 +83,29 @@ def timed(lat, fn, reps):
     return s.elapsed_time(e) / reps
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stream_fp64_dw_32x32x32x32.json")
-
-
-def measured_traffic(L):
-    """HBM bytes per FP64 D_W launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (calibrated by a 1 GiB copy); None if absent or for
-    another volume."""
+def measured_traffic(name, L, key="traffic_bytes_per_launch", nrhs=None):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/<name>.json, calibrated by a 1 GiB copy);
+    None if absent or measured on another workload."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(os.path.join(ROOT, "profiles", name + ".json")) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None, None
-    if tuple(L) != (32, 32, 32, 32):
+    if tuple(L) != (32, 32, 32, 32) or (nrhs is not None and t.get("nrhs") != nrhs) or key not in t:
         return None, None
-    return float(t["traffic_bytes_per_launch"]), "profiles/k_stream_fp64_dw_32x32x32x32.md"
+    return float(t[key]), f"profiles/{name}.md"
+
+
+def inner_kernel_bytes(precision, n):
+    """Algorithmic bytes per site of one launch of each inner stencil over n
+    RHS (links once per launch): D with |Dp|^2 reads p and writes t; D^dagger
+    with the fused update reads t, reads and writes r, and once in four
+    (averaged here) reads the four p of the group and reads and writes the
+    FP32 x_inner (192 B)."""
+    links = 288.0 if precision == "f32" else 144.0
+    spinor = 52.0 if precision == "h16" else 96.0
+    return links + n * 2 * spinor, links + n * (3 * spinor + (4 * spinor + 192.0) / 4)
 
 
 def lo_dslash_bytes(precision, n):
@@ -122,20 +133,45 @@ def copy_calibration(lat):
     return ms
 
 
-def cpu_baseline(U, b, L, kappa, iters, gpu_iterations):
-    """Oracle (C/OpenMP restatement of the reference CPU algorithm) on the
-    same inputs: a bounded sample of `iters` FP64 CG iterations."""
+def host_cpu():
+    """CPU model, sockets and physical cores of the host (lscpu)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        cores = int(info["Core(s) per socket"]) * int(info["Socket(s)"])
+    except (KeyError, ValueError):
+        cores = None
+    return {"model": info.get("Model name", "unknown"), "physical_cores": cores,
+            "logical_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(U, b, L, kappa, rtol, maxiter, gpu_iterations):
+    """The same FP64 solve (zero start, rtol on the recursive residual, as
+    cg!) by the reference's own CPU algorithm, restated in C with OpenMP:
+    LatticeMatrix padded storage (nw = 1), set_halo! before every operator
+    application, the half-spin kernel reading neighbours from the halo,
+    unfused BLAS-1 and per-thread reductions (the JACC threads backend).  One
+    whole solve is timed (setup excluded); threads = OMP_NUM_THREADS."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     threads = orc.num_threads()
-    op = orc.NormalOp(orc.OP_DDAGD_WILSON, L, U=U, kappa=kappa)
-    t0 = time.perf_counter()
-    _, res = orc.cg(op, b, rtol=1e-30, maxiter=iters, order=1)
-    dt = time.perf_counter() - t0
-    per_iter = dt / max(res.iterations, 1)
-    return {"value": per_iter * gpu_iterations, "unit": "s/RHS", "cores": threads, "kind": "port",
-            "sample": f"{res.iterations} FP64 CG iterations of the same 32^4 solve on the C/OpenMP oracle "
-                      f"({threads} threads, {per_iter:.3f} s/iteration) x {gpu_iterations} iterations"}
+    host = host_cpu()
+    x, res, dt = orc.refcpu_cg_ddagd(U, b, L, kappa, rtol=rtol, maxiter=maxiter)
+    return {"value": dt, "unit": "s/RHS", "cores": threads, "kind": "port",
+            "sample": f"one whole FP64 D^dagger D CG solve of the benchmark's RHS 0 (32^4, kappa {kappa}, rtol "
+                      f"{rtol:g}, zero start): {res.iterations} iterations ({'converged' if res.converged else res.reason}; "
+                      f"the GPU FP64 CG took {gpu_iterations}) on the reference's CPU algorithm restated in C/OpenMP "
+                      f"(padded storage, halo refresh per application, unfused BLAS-1), {threads} threads on "
+                      f"{host['model']} ({host['physical_cores']} physical cores, {host['affinity_cpus']} CPUs "
+                      f"available to this process)",
+            "host": host, "iterations": res.iterations}, x
 
 
 def require_converged(name, results, rtol):
@@ -191,14 +227,18 @@ def main():
         bytes_dm = lo_dslash_bytes(args.precision, n)
         log(f"bench: {args.precision} fused D / D^dagger pair x{n}: {tune['ms_per_apply'] * 1e3:.1f} us per application "
             f"per RHS, {bytes_dm * V / (tune['ms_per_apply'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
-        # eight inner iterations of the mixed solve at the tuned shape: the
-        # last dispatches of the 16-bit stencils in a profile of this run
-        xm = lat.spinor(n)
-        lmx.cg_batched_(xm, D, bg, precision=args.precision, rtol=args.rtol, maxiter=8, use_graph=False)
+        # the two inner stencils launched on their own at the tuned shape, five
+        # groups of four inner iterations (the first warms up): the dispatches
+        # a profile of this run attributes to them
+        kt = lmx.inner_kernel_times(D, args.precision, n, tune["rhs_block"], groups=4)
+        bD, bX = inner_kernel_bytes(args.precision, n)
+        log(f"bench: {args.precision} inner D {kt['D']:.1f} us ({bD * V / (kt['D'] * 1e-6) / 1e9:.0f} GB/s), "
+            f"D^dagger with the update {kt['Ddag']:.1f} us ({bX * V / (kt['Ddag'] * 1e-6) / 1e9:.0f} GB/s) per launch")
         torch.cuda.synchronize()
         print(json.dumps({"mode": "kernels", "fp64_dslash_us": t_d64 * 1e3, "fp64_dslash_gbs": ach_d64,
                           "copy_1gib_us": t_copy * 1e3, "lo_dslash_us_per_rhs": tune["ms_per_apply"] * 1e3,
-                          "lo_rhs_block": tune["rhs_block"]}), flush=True)
+                          "lo_rhs_block": tune["rhs_block"], "inner_D_us": kt["D"], "inner_Ddag_us": kt["Ddag"]}),
+              flush=True)
         return
 
     # ---- plain FP64 CG (the reference algorithm), RHS 0 -------------------
@@ -225,6 +265,9 @@ def main():
     log(f"bench: autotuned {args.precision} fused D / D^dagger pair x{n}: TL {(blk >> 12) & 0xff}, z-ranges "
         f"{blk >> 20} (0: automatic), {(blk >> 4) & 0xff} RHS per launch, {blk & 15} per work item: "
         f"{tune['ms_per_apply'] * 1e3:.1f} us per application per RHS, {ach_dm:.0f} GB/s algorithmic")
+    kt = lmx.inner_kernel_times(D, args.precision, n, tune["rhs_block"], groups=4)
+    bD, bX = inner_kernel_bytes(args.precision, n)
+    log(f"bench: {args.precision} inner D {kt['D']:.1f} us, D^dagger with the update {kt['Ddag']:.1f} us per launch")
     xm = lat.spinor(n)
 
     def solve_mixed():
@@ -279,11 +322,33 @@ def main():
             f"iterations {min(a['iterations'])}-{max(a['iterations'])}; max true rel residual "
             f"{a['true_relative_residual_max']:.3e}; RHS0 error vs FP64 {erra:.3e}")
 
-    traffic, traffic_src = measured_traffic(L)
+    traffic, traffic_src = measured_traffic("k_stream_fp64_dw_32x32x32x32", L)
+    pair_name = f"k_stream_{args.precision}_pair_32x32x32x32_{n}rhs"
+    tr_D, tr_src = measured_traffic(pair_name, L, "D_traffic_bytes_per_launch", n)
+    tr_X, _ = measured_traffic(pair_name, L, "Ddag_traffic_bytes_per_launch", n)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(U, bg.rhs(0).to_numpy(), L, args.kappa, args.cpu_iters, res64.iterations)
-        log(f"bench: cpu baseline {cpu['sample']}")
+        log("bench: cpu baseline: one FP64 solve on the reference's CPU algorithm ...")
+        cpu, xc = cpu_baseline(U, bg.rhs(0).to_numpy(), L, args.kappa, args.rtol, args.cpu_maxiter, res64.iterations)
+        cpu["rhs0_error_vs_gpu_fp64"] = float(np.linalg.norm(xc - x64) / np.linalg.norm(x64))
+        log(f"bench: cpu baseline {cpu['value']:.1f} s: {cpu['sample']}; error vs the GPU FP64 solution "
+            f"{cpu['rhs0_error_vs_gpu_fp64']:.2e}")
+
+    def roofline(kernel, bytes_per_launch, us, tr, src):
+        ach = bytes_per_launch / (us * 1e-6) / 1e9
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": tr, "kernel": kernel, "bytes_per_launch": bytes_per_launch, "us_per_launch": us,
+                "traffic_source": src}
+
+    lo_name = {"h16": "SpH16In, LinkF16, SpH16Out", "f16": "SpF32In, LinkF16, SpF32Out",
+               "f32": "SpF32In, LinkF32, SpF32Out"}[args.precision]
+    roof_X = roofline(f"k_stream<{lo_name}, ..., XrEx, +1, 0, NR, 3> (D^dagger with the r / x update, {n} RHS, "
+                      f"the dominant kernel of the timed solve)", bX * V, kt["Ddag"], tr_X, tr_src)
+    roof_D = roofline(f"k_stream<{lo_name}, ..., NoEx, -1, 0, NR, 2> (D with |Dp|^2, {n} RHS)", bD * V, kt["D"],
+                      tr_D, tr_src)
+    roof_64 = roofline("k_stream<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, NoEx, -1, 0, 1, 0, false> "
+                       "(FP64 D_W, 32^4; the reliable updates and the FP64 CG)", bytes_d64, t_d64 * 1e3, traffic,
+                       traffic_src)
 
     out = {
         "metric": "D^dagger D CG time-to-solution/RHS @1e-10 (mixed+multiRHS vs FP64); dslash HBM GB/s",
@@ -317,12 +382,8 @@ def main():
                          "rhs_block": tune["rhs_block"], "us_per_rhs": tune["ms_per_apply"] * 1e3,
                          "algorithmic_bytes_per_site_per_rhs": bytes_dm, "achieved_gbs": ach_dm,
                          "frac": ach_dm / HBM_PEAK_GBS},
-        "roofline": {"bound": "hbm", "achieved": ach_d64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_d64 / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stream<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, NoEx, -1, 0, 1, 0, false> "
-                               "(FP64 D_W, 32^4)",
-                     "bytes_per_launch": bytes_d64, "us_per_launch": t_d64 * 1e3,
-                     "traffic_source": traffic_src},
+        "roofline": roof_X,
+        "roofline_kernels": {"inner_Ddag": roof_X, "inner_D": roof_D, "fp64_dw": roof_64},
         "alt_precision": alt,
         "cpu_baseline": cpu,
     }

@@ -123,6 +123,23 @@ int lmx_abi_version(void);
  * stream).  Replaces the single-rank construction of LatticeMatrix /
  * LatticeMatrix_standard (src/LatticeMatrices_core.jl:297-307). */
 int lmx_context_create(lmx_context** ctx, const int32_t dims[4], const double* phases, void* stream);
+/* The same for fields of nc colours: LatticeMatrix{4,T,AT,NC1,...} with
+ * NC1 = nc (src/LatticeMatrices_core.jl:297-307).  nc = 3 is
+ * lmx_context_create; nc = 2 runs the reference's generic-NC kernels
+ * (kernel_WilsonDiracOperator4D! and its adjoint / Donly variants,
+ * src/Operators/WilsonDiracOperator.jl:80-147,444-511,554-621,665-733; the
+ * generic clover field strength and term,
+ * src/Operators/WilsonDiracCloverOperator.jl:57-118,481-525; the generic link
+ * pullback, ext/AD/WilsonDiracOperator.jl:27-53) in their operation order.
+ * On such a context spinors hold 4 nc complex components per site (component
+ * spin*nc + colour), links nc*nc entries (row*nc + col), the clover cache six
+ * full nc x nc planes; import / export, BLAS-1, lmx_wilson_apply(_padded),
+ * lmx_ddagd_apply, lmx_cg_ddagd(_noalloc), lmx_cg_generic, lmx_clover_export,
+ * lmx_wilson_pullback (Wilson operator) and lmx_wilson_force (F = NULL) work
+ * on it; the single-precision, mixed and batched paths, the autotuner, the
+ * TA projection, the clover link pullback and lmx_spinor_randomize return
+ * LMX_ERR_UNSUPPORTED.  Other nc: LMX_ERR_UNSUPPORTED. */
+int lmx_context_create_nc(lmx_context** ctx, const int32_t dims[4], const double* phases, void* stream, int nc);
 int lmx_context_destroy(lmx_context* ctx);
 int lmx_context_set_stream(lmx_context* ctx, void* stream);
 int lmx_context_synchronize(lmx_context* ctx);
@@ -236,6 +253,17 @@ void lmx_cg_params_default(lmx_cg_params* p);
 int lmx_cg_ddagd(lmx_wilson* op, void* x, const void* b, void* r, void* p, void* Ap, void* tmp,
                  const lmx_cg_params* params, lmx_cg_result* result);
 
+/* cg!(x, DdagDOp(D, temp), b, r, p, Ap; rtol, atol, maxiter)
+ * (src/LinearAlgebras/cg.jl:73-137) with x and b the LatticeMatrix.A arrays
+ * themselves (complex128, column-major [nc,4,L1+2nw,...,L4+2nw]): the solve
+ * runs on the library layout in `work` (at least lmx_spinor_bytes(ctx,
+ * LMX_F64, 6) bytes, caller-owned; LMX_ERR_WORKSPACE if smaller), x is the
+ * initial guess and the solution, and on return x's halo holds what set_halo!
+ * would put there.  The reference's r, p, Ap and temp have no role here (the
+ * work fields are in `work`).  Synchronous. */
+int lmx_cg_ddagd_padded(lmx_wilson* op, void* x, const void* b, int nw, void* work, size_t work_bytes,
+                        const lmx_cg_params* params, lmx_cg_result* result);
+
 /* No counterpart in the reference: lmx_cg_ddagd for callers that cannot
  * allocate device memory (code between graph captures, tight serving or HMC
  * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
@@ -313,6 +341,16 @@ typedef struct {
  * lattice is ignored and retuned) and use it for later lmx_cg_batched calls
  * with rhs_block = 0. */
 int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char* cache_path, lmx_tune_result* out);
+
+/* Average duration in microseconds of the mixed solver's two inner stencil
+ * kernels at launch shape rhs_block (0: the shape lmx_autotune chose, or the
+ * default), over `groups` groups of four inner iterations as the solver runs
+ * them, each launch bracketed by HIP events on the context stream:
+ * us[0] = D with |D p|^2 (one launch over nrhs RHS), us[1] = D^dagger with
+ * the r update, the x update deferred three times in four and applied the
+ * fourth.  Measurement only (used for the roofline of these kernels); it
+ * allocates its buffers. */
+int lmx_inner_kernel_times(lmx_wilson* op, int precision, int nrhs, int rhs_block, int groups, double us[2]);
 
 /* ======================================================================= */
 /* Site random numbers and lattice fills                                    */

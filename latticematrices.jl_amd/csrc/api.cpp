@@ -39,6 +39,20 @@ static int check_op(const lmx_wilson* op) {
     return LMX_OK;
 }
 
+// Entry points built for NC = 3 fields only (the low-precision and mixed paths,
+// the TA projection, the spinor fills).
+static int check_nc3(const lmx_context* ctx, const char* what) {
+    LMX_CHECK(ctx->nc == 3, LMX_ERR_UNSUPPORTED, "%s is built for NC = 3 fields (this context has NC = %d)", what,
+              ctx->nc);
+    return LMX_OK;
+}
+static int check_nw_fits(const lmx_context* ctx, int nw) {
+    for (int d = 0; d < 4; ++d)
+        LMX_CHECK(nw <= ctx->g.L[d], LMX_ERR_ARG, "halo width nw=%d exceeds local lattice size %d in dimension %d", nw,
+                  ctx->g.L[d], d + 1);
+    return LMX_OK;
+}
+
 // _validate_cg_arguments, src/LinearAlgebras/cg.jl:33-47
 static int validate_cg(const void* const* fields, const char* const* names, int n, const lmx_cg_params* p) {
     for (int i = 0; i < n; ++i) LMX_CHECK(fields[i] != nullptr, LMX_ERR_ARG, "CG field %s is NULL", names[i]);
@@ -161,7 +175,12 @@ static int cg_ddagd_run(lmx_wilson* op, lmx_context* c, void* x, const void* b, 
                   c->host_state->iteration);
         for (int k = 0; k < chunk; ++k) {
             LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, T, P, 1, 0, gate));
-            LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, AP, T, 1, LMX_DAGGER, pap));
+            if (c->nc == 3) {  // p.Ap fused into D^dagger's epilogue
+                LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, AP, T, 1, LMX_DAGGER, pap));
+            } else {
+                LMX_TRY(launch_wilson_f64(c, U, F, op->kappa, op->csw, AP, T, 1, LMX_DAGGER, gate));
+                LMX_TRY(launch_dot(c, P, AP, FIN_PAP, st, nullptr));
+            }
             LMX_TRY(launch_cg_update_xr(c, X, R, P, AP, st));
             LMX_TRY(launch_cg_update_p(c, R, P, st));
         }
@@ -173,7 +192,10 @@ static int cg_ddagd_run(lmx_wilson* op, lmx_context* c, void* x, const void* b, 
 
 static int refresh_derived(lmx_wilson* op) {
     lmx_context* ctx = op->ctx;
-    if (op->csw != 0.0) {
+    if (op->csw != 0.0 && ctx->nc != 3) {
+        if (!op->clover64) LMX_HIP(hipMalloc(&op->clover64, 6 * (size_t)link_nelem(ctx) * ctx->V * sizeof(double2)));
+        LMX_TRY(launch_clover_build_nc(ctx, op->links, op->clover64));
+    } else if (op->csw != 0.0) {
         if (!op->clover64) LMX_HIP(hipMalloc(&op->clover64, 27 * (size_t)ctx->V * sizeof(double2)));
         LMX_TRY(launch_clover_build_f64(ctx, op->links, op->clover64));
     }
@@ -205,9 +227,16 @@ void lmx_cg_params_default(lmx_cg_params* p) {
 }
 
 int lmx_context_create(lmx_context** out, const int32_t dims[4], const double* phases, void* stream) {
+    return lmx_context_create_nc(out, dims, phases, stream, 3);
+}
+
+int lmx_context_create_nc(lmx_context** out, const int32_t dims[4], const double* phases, void* stream, int nc) {
     clear_error();
     LMX_CHECK(out != nullptr && dims != nullptr, LMX_ERR_ARG, "NULL argument");
     *out = nullptr;
+    LMX_CHECK(nc >= 1, LMX_ERR_ARG, "NC must be positive, got %d", nc);
+    LMX_CHECK(nc == 2 || nc == 3, LMX_ERR_UNSUPPORTED, "fields of NC = %d colours are not built (NC = 2 and 3 are)",
+              nc);
     int64_t V = 1;
     for (int d = 0; d < 4; ++d) {
         LMX_CHECK(dims[d] > 0, LMX_ERR_ARG, "global lattice sizes must be positive, got %d in dimension %d", dims[d],
@@ -242,6 +271,7 @@ int lmx_context_create(lmx_context** out, const int32_t dims[4], const double* p
         if (!g.phase_one[d] && !g.phase_neg[d]) g.general = 1;
     g.V = (int)V;
     c->V = V;
+    c->nc = nc;
     c->stream = (hipStream_t)stream;
     const char* tl = getenv("LMX_STREAM_TL");
     const char* zs = getenv("LMX_STREAM_ZS");
@@ -313,19 +343,21 @@ int64_t lmx_context_volume(const lmx_context* c) { return c ? c->V : -1; }
 size_t lmx_spinor_bytes(const lmx_context* c, int precision, int nrhs) {
     if (!c) return 0;
     // LMX_H16: 12 16-bit (re, im) pairs plus one float scale per site
-    size_t e = precision == LMX_F64 ? 12 * sizeof(double2) : precision == LMX_F32 ? 12 * sizeof(float2)
-             : precision == LMX_H16 ? 12 * sizeof(int) + sizeof(float) : 12 * sizeof(__half2);
+    const size_t n = (size_t)spinor_ncomp(c);
+    size_t e = precision == LMX_F64 ? n * sizeof(double2) : precision == LMX_F32 ? n * sizeof(float2)
+             : precision == LMX_H16 ? n * sizeof(int) + sizeof(float) : n * sizeof(__half2);
     return e * (size_t)c->V * (size_t)std::max(nrhs, 0);
 }
 size_t lmx_links_bytes(const lmx_context* c, int precision) {
     if (!c) return 0;
     size_t e = precision == LMX_F64 ? sizeof(double2) : precision == LMX_F32 ? sizeof(float2) : sizeof(__half2);  // H16: FP16 links
-    return e * 36 * (size_t)c->V;
+    return e * 4 * (size_t)link_nelem(c) * (size_t)c->V;
 }
 size_t lmx_clover_bytes(const lmx_context* c, int precision) {
     if (!c) return 0;
     size_t e = precision == LMX_F64 ? sizeof(double2) : precision == LMX_F32 ? sizeof(float2) : sizeof(__half2);
-    return e * 27 * (size_t)c->V;
+    // NC = 3: the 27 complex slots of the compact planes; other NC: 6 full planes
+    return e * (c->nc == 3 ? 27 : 6 * (size_t)link_nelem(c)) * (size_t)c->V;
 }
 
 int lmx_spinor_import(lmx_context* c, void* dst, const void* src, int nw, int nrhs) {
@@ -359,6 +391,7 @@ int lmx_spinor_randomize(lmx_context* c, void* dst, int nrhs, int dist, const lm
                          int algorithm, const int32_t* global_offset, const int32_t* global_dims) {
     clear_error();
     LMX_TRY(check_ctx(c));
+    LMX_TRY(check_nc3(c, "the spinor fill"));
     return launch_spinor_randomize(c, dst, nrhs, dist, keys, sigma, algorithm, global_offset, global_dims);
 }
 int lmx_field_pad(lmx_context* c, void* dst, const void* src, int elem, int nc1, int nc2, int nw) {
@@ -388,7 +421,7 @@ int lmx_axpby(lmx_context* c, double a, const void* X, double b, void* Y, int nr
     clear_error();
     LMX_TRY(check_ctx(c));
     LMX_CHECK(X && Y && nrhs >= 0, LMX_ERR_ARG, "bad axpby arguments");
-    return launch_axpby(c, a, (const double2*)X, b, (double2*)Y, 12 * c->V * (int64_t)nrhs);
+    return launch_axpby(c, a, (const double2*)X, b, (double2*)Y, spinor_ncomp(c) * c->V * (int64_t)nrhs);
 }
 
 int lmx_dot(lmx_context* c, const void* A, const void* B, double out[2]) {
@@ -411,7 +444,7 @@ int lmx_scale(lmx_context* c, void* out, const void* in, double sr, double si, i
     clear_error();
     LMX_TRY(check_ctx(c));
     LMX_CHECK(out && in && nrhs >= 0, LMX_ERR_ARG, "bad scale arguments");
-    return launch_scale(c, (double2*)out, (const double2*)in, sr, si, 12 * c->V * (int64_t)nrhs);
+    return launch_scale(c, (double2*)out, (const double2*)in, sr, si, spinor_ncomp(c) * c->V * (int64_t)nrhs);
 }
 
 int lmx_wilson_create(lmx_context* c, lmx_wilson** out, const void* links, double kappa, double csw) {
@@ -468,6 +501,7 @@ int lmx_clover_export(lmx_wilson* op, void* F_out) {
     LMX_TRY(check_op(op));
     LMX_CHECK(F_out != nullptr, LMX_ERR_ARG, "F_out is NULL");
     LMX_CHECK(op->clover64 != nullptr, LMX_ERR_ARG, "operator has no clover term (csw == 0)");
+    if (op->ctx->nc != 3) return launch_clover_export_nc(op->ctx, op->clover64, (double2*)F_out);
     return launch_clover_export(op->ctx, op->clover64, (double2*)F_out);
 }
 
@@ -486,6 +520,7 @@ int lmx_wilson_apply_f32(lmx_wilson* op, void* out, const void* in, int flags, i
     clear_error();
     LMX_TRY(check_op(op));
     LMX_CHECK(out && in && nrhs >= 0, LMX_ERR_ARG, "bad wilson_apply_f32 arguments");
+    LMX_TRY(check_nc3(op->ctx, "the single-precision apply"));
     LMX_CHECK(out != in, LMX_ERR_ALIAS, "output and input of the Wilson operator must not alias");
     LMX_CHECK((flags & ~(LMX_DAGGER | LMX_DONLY)) == 0, LMX_ERR_ARG, "unknown flags %d", flags);
     if (nrhs == 0 || op->ctx->V == 0) return LMX_OK;
@@ -513,12 +548,19 @@ int lmx_wilson_apply_padded(lmx_wilson* op, void* C, const void* psi, int nw, in
     LMX_CHECK(C && psi && nrhs >= 0 && nw >= 0, LMX_ERR_ARG, "bad wilson_apply_padded arguments");
     LMX_CHECK(C != psi, LMX_ERR_ALIAS, "output and input of the Wilson operator must not alias");
     LMX_CHECK((flags & ~(LMX_DAGGER | LMX_DONLY)) == 0, LMX_ERR_ARG, "unknown flags %d", flags);
+    if (op->ctx->nc != 3) {
+        LMX_TRY(check_nw_fits(op->ctx, nw));
+        return launch_wilson_nc(op->ctx, op->links, (flags & LMX_DONLY) ? nullptr : op->clover64, op->kappa, op->csw,
+                                (double2*)C, (const double2*)psi, nrhs, flags, nw, nullptr, nullptr);
+    }
     return launch_wilson_padded(op->ctx, op->links, (flags & LMX_DONLY) ? nullptr : op->clover64, op->kappa, op->csw,
                                 (double2*)C, (const double2*)psi, nw, nrhs, flags);
 }
 
 int lmx_wilson_apply_padded_f32(lmx_wilson* op, void* C, const void* psi, int nw, int flags, int nrhs) {
     clear_error();
+    LMX_TRY(check_op(op));
+    LMX_TRY(check_nc3(op->ctx, "the single-precision apply"));
     LMX_TRY(check_op(op));
     LMX_CHECK(C && psi && nrhs >= 0 && nw >= 0, LMX_ERR_ARG, "bad wilson_apply_padded_f32 arguments");
     LMX_CHECK(C != psi, LMX_ERR_ALIAS, "output and input of the Wilson operator must not alias");
@@ -577,6 +619,30 @@ int lmx_cg_ddagd_noalloc(lmx_wilson* op, void* x, const void* b, void* r, void* 
     return cg_ddagd_run(op, &local, x, b, r, p, Ap, tmp, params, result);
 }
 
+int lmx_cg_ddagd_padded(lmx_wilson* op, void* x, const void* b, int nw, void* work, size_t work_bytes,
+                        const lmx_cg_params* params, lmx_cg_result* result) {
+    clear_error();
+    LMX_TRY(check_op(op));
+    const void* f[2] = {x, b};
+    const char* names[2] = {"x", "b"};
+    LMX_TRY(validate_cg(f, names, 2, params));
+    LMX_CHECK(result != nullptr && work != nullptr, LMX_ERR_ARG, "bad cg_ddagd_padded arguments");
+    lmx_context* c = op->ctx;
+    const size_t field = lmx_spinor_bytes(c, LMX_F64, 1);
+    LMX_CHECK(work_bytes >= 6 * field, LMX_ERR_WORKSPACE, "workspace of %zu bytes is below the %zu of six spinor fields",
+              work_bytes, 6 * field);
+    unsigned char* w = (unsigned char*)work;
+    void *X = w, *B = w + field, *R = w + 2 * field, *P = w + 3 * field, *AP = w + 4 * field, *T = w + 5 * field;
+    // the LatticeMatrix arrays -> the library layout; the solve; x back with
+    // its halo refilled as set_halo! would (b is only read)
+    LMX_TRY(spinor_import(c, X, x, nw, 1));
+    LMX_TRY(spinor_import(c, B, b, nw, 1));
+    LMX_TRY(cg_ddagd_run(op, c, X, B, R, P, AP, T, params, result));
+    LMX_TRY(spinor_export(c, x, nw, X, 1));
+    LMX_HIP(hipStreamSynchronize(c->stream));
+    return LMX_OK;
+}
+
 int lmx_cg_generic(lmx_context* c, lmx_apply_fn A, void* A_user, void* x, const void* b, void* r, void* p, void* Ap,
                    const lmx_cg_params* params, lmx_cg_result* result) {
     clear_error();
@@ -615,6 +681,7 @@ int lmx_cg_batched_workspace(lmx_wilson* op, int nrhs, const lmx_cg_params* para
     clear_error();
     LMX_TRY(check_op(op));
     LMX_CHECK(bytes && params && nrhs >= 0, LMX_ERR_ARG, "bad workspace query");
+    LMX_TRY(check_nc3(op->ctx, "the batched solver"));
     *bytes = cg_batched_workspace_impl(op, nrhs, params);
     return LMX_OK;
 }
@@ -624,6 +691,7 @@ int lmx_cg_batched(lmx_wilson* op, void* x, const void* b, int nrhs, void* work,
     clear_error();
     LMX_TRY(check_op(op));
     LMX_CHECK(x && b && results && params, LMX_ERR_ARG, "bad cg_batched arguments");
+    LMX_TRY(check_nc3(op->ctx, "the batched solver"));
     LMX_CHECK(x != b, LMX_ERR_ALIAS, "CG fields x and b must not alias");
     LMX_CHECK(nrhs >= 0 && nrhs <= kMaxStates, LMX_ERR_ARG, "nrhs must be in [0, %d]", kMaxStates);
     const void* f[2] = {x, b};
@@ -636,6 +704,10 @@ namespace lmx {
 // Hopping pullback, plus the clover part (coefficient * csw) for a clover operator.
 static int operator_pullback(lmx_wilson* op, double2* dU, const double2* dres, const double2* psi,
                              double coefficient) {
+    if (op->ctx->nc != 3) {
+        LMX_CHECK(op->csw == 0.0, LMX_ERR_UNSUPPORTED, "the clover link pullback is built for NC = 3 fields");
+        return launch_pullback_nc(op->ctx, dU, dres, psi, coefficient);
+    }
     LMX_TRY(launch_pullback_f64(op->ctx, dU, dres, psi, coefficient));
     if (op->csw == 0.0) return LMX_OK;
     if (!op->clover_dF) LMX_HIP(hipMalloc(&op->clover_dF, 6 * 9 * (size_t)op->ctx->V * sizeof(double2)));
@@ -654,6 +726,7 @@ int lmx_ta_project(lmx_context* c, void* F, const void* links, const void* G) {
     clear_error();
     LMX_TRY(check_ctx(c));
     LMX_CHECK(F && links && G, LMX_ERR_ARG, "bad ta_project arguments");
+    LMX_TRY(check_nc3(c, "the traceless anti-Hermitian projection"));
     return launch_ta_project(c, (double2*)F, (const double2*)links, (const double2*)G);
 }
 
@@ -663,7 +736,8 @@ int lmx_wilson_force(lmx_wilson* op, void* G, void* F, const void* phi, void* et
     LMX_TRY(check_op(op));
     LMX_CHECK(G && phi && eta && chi && work && params && result, LMX_ERR_ARG, "bad force arguments");
     lmx_context* c = op->ctx;
-    const size_t fb = 12 * (size_t)c->V;
+    if (F) LMX_TRY(check_nc3(c, "the traceless anti-Hermitian projection"));
+    const size_t fb = (size_t)spinor_ncomp(c) * (size_t)c->V;
     double2* w = (double2*)work;
     double2 *r = w, *p = w + fb, *Ap = w + 2 * fb, *tmp = w + 3 * fb;
     LMX_TRY(launch_zero(c, (double2*)eta, (int64_t)fb));
@@ -673,7 +747,7 @@ int lmx_wilson_force(lmx_wilson* op, void* G, void* F, const void* phi, void* et
                               e));
     // dresult = -2 chi (the dot pullback of g = -2 Re dot(chi, D eta))
     LMX_TRY(launch_scale(c, tmp, (const double2*)chi, -2.0, 0.0, (int64_t)fb));
-    LMX_TRY(launch_zero(c, (double2*)G, 36 * (int64_t)c->V));
+    LMX_TRY(launch_zero(c, (double2*)G, 4 * (int64_t)link_nelem(c) * c->V));
     LMX_TRY(operator_pullback(op, (double2*)G, tmp, (const double2*)eta, -op->kappa));
     if (F) LMX_TRY(launch_ta_project(c, (double2*)F, op->links, (const double2*)G));
     LMX_HIP(hipStreamSynchronize(c->stream));

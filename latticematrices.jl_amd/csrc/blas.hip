@@ -121,6 +121,7 @@ int launch_finalize(lmx_context* ctx, int nb, int kind, bool want_im, CGState* s
 }
 
 // dot(A, B) = sum conj(A) B
+template <int NCOMP>
 __global__ void __launch_bounds__(256) k_dot(const double2* __restrict__ A, const double2* __restrict__ B, int V,
                                              double* partials, int want_im, CGState* st) {
     if (st != nullptr && st->done) return;
@@ -130,7 +131,7 @@ __global__ void __launch_bounds__(256) k_dot(const double2* __restrict__ A, cons
     if (s < V) {
         cx<double> acc = mk<double>(0.0, 0.0);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
+        for (int k = 0; k < NCOMP; ++k) {
             double2 a = A[(long)k * V + s], b = B[(long)k * V + s];
             acc = cadd(acc, cmul(cconj(mk<double>(a.x, a.y)), mk<double>(b.x, b.y)));
         }
@@ -157,6 +158,7 @@ __global__ void __launch_bounds__(256) k_scale(double2* __restrict__ out, const 
 }
 
 // r = b - Ap (axpby!(1,b,0,r); axpby!(-1,Ap,1,r)); p = r; rr = |r|^2  (cg.jl:91-95)
+template <int NCOMP>
 __global__ void __launch_bounds__(256) k_cg_setup_r(const double2* __restrict__ b, const double2* __restrict__ Ap,
                                                     double2* __restrict__ r, double2* __restrict__ p, int V,
                                                     double* partials, CGState* st) {
@@ -167,7 +169,7 @@ __global__ void __launch_bounds__(256) k_cg_setup_r(const double2* __restrict__ 
     if (s < V) {
         cx<double> acc = mk<double>(0.0, 0.0);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
+        for (int k = 0; k < NCOMP; ++k) {
             const long i = (long)k * V + s;
             double2 bv = b[i], av = Ap[i];
             double2 rv = make_double2(-1.0 * av.x + bv.x, -1.0 * av.y + bv.y);
@@ -182,6 +184,7 @@ __global__ void __launch_bounds__(256) k_cg_setup_r(const double2* __restrict__ 
 }
 
 // x = alpha*p + x; r = (-alpha)*Ap + r; rr_new = |r|^2  (cg.jl:116-120)
+template <int NCOMP>
 __global__ void __launch_bounds__(256) k_cg_update_xr(double2* __restrict__ x, double2* __restrict__ r,
                                                       const double2* __restrict__ p, const double2* __restrict__ Ap,
                                                       int V, double* partials, CGState* st) {
@@ -194,7 +197,7 @@ __global__ void __launch_bounds__(256) k_cg_update_xr(double2* __restrict__ x, d
     if (s < V) {
         cx<double> acc = mk<double>(0.0, 0.0);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
+        for (int k = 0; k < NCOMP; ++k) {
             const long i = (long)k * V + s;
             double2 pv = p[i], xv = x[i], av = Ap[i], rv = r[i];
             xv = make_double2(alpha * pv.x + 1.0 * xv.x, alpha * pv.y + 1.0 * xv.y);
@@ -210,6 +213,7 @@ __global__ void __launch_bounds__(256) k_cg_update_xr(double2* __restrict__ x, d
 }
 
 // p = 1*r + beta*p  (cg.jl:130-131)
+template <int NCOMP>
 __global__ void __launch_bounds__(256) k_cg_update_p(const double2* __restrict__ r, double2* __restrict__ p, int V,
                                                      CGState* st) {
     if (st->done) return;
@@ -217,7 +221,7 @@ __global__ void __launch_bounds__(256) k_cg_update_p(const double2* __restrict__
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s < V) {
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
+        for (int k = 0; k < NCOMP; ++k) {
             const long i = (long)k * V + s;
             double2 rv = r[i], pv = p[i];
             p[i] = make_double2(1.0 * rv.x + beta * pv.x, 1.0 * rv.y + beta * pv.y);
@@ -246,8 +250,12 @@ static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)
 int launch_dot(lmx_context* ctx, const double2* A, const double2* B, int fin_kind, CGState* st, double* out2) {
     const int nb = nblocks(ctx->V);
     const bool want_im = fin_kind == FIN_OUT;
-    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(256), 0, ctx->stream, A, B, (int)ctx->V, ctx->partials, want_im ? 1 : 0,
-                       st);
+    if (ctx->nc == 3)
+        hipLaunchKernelGGL(k_dot<12>, dim3(nb), dim3(256), 0, ctx->stream, A, B, (int)ctx->V, ctx->partials,
+                           want_im ? 1 : 0, st);
+    else
+        hipLaunchKernelGGL(k_dot<8>, dim3(nb), dim3(256), 0, ctx->stream, A, B, (int)ctx->V, ctx->partials,
+                           want_im ? 1 : 0, st);
     LMX_HIP(hipGetLastError());
     return launch_finalize(ctx, nb, fin_kind, want_im, st, out2);
 }
@@ -283,21 +291,32 @@ int launch_scale(lmx_context* ctx, double2* out, const double2* in, double sr, d
 }
 
 int launch_cg_setup_r(lmx_context* ctx, const double2* b, const double2* Ap, double2* r, double2* p, CGState* st) {
-    hipLaunchKernelGGL(k_cg_setup_r, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, b, Ap, r, p, (int)ctx->V,
-                       ctx->partials, st);
+    if (ctx->nc == 3)
+        hipLaunchKernelGGL(k_cg_setup_r<12>, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, b, Ap, r, p,
+                           (int)ctx->V, ctx->partials, st);
+    else
+        hipLaunchKernelGGL(k_cg_setup_r<8>, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, b, Ap, r, p,
+                           (int)ctx->V, ctx->partials, st);
     LMX_HIP(hipGetLastError());
     return launch_finalize(ctx, nblocks(ctx->V), FIN_RR0, false, st, nullptr);
 }
 
 int launch_cg_update_xr(lmx_context* ctx, double2* x, double2* r, const double2* p, const double2* Ap, CGState* st) {
-    hipLaunchKernelGGL(k_cg_update_xr, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, x, r, p, Ap, (int)ctx->V,
-                       ctx->partials, st);
+    if (ctx->nc == 3)
+        hipLaunchKernelGGL(k_cg_update_xr<12>, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, x, r, p, Ap,
+                           (int)ctx->V, ctx->partials, st);
+    else
+        hipLaunchKernelGGL(k_cg_update_xr<8>, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, x, r, p, Ap,
+                           (int)ctx->V, ctx->partials, st);
     LMX_HIP(hipGetLastError());
     return launch_finalize(ctx, nblocks(ctx->V), FIN_RRNEW, false, st, nullptr);
 }
 
 int launch_cg_update_p(lmx_context* ctx, const double2* r, double2* p, CGState* st) {
-    hipLaunchKernelGGL(k_cg_update_p, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, r, p, (int)ctx->V, st);
+    if (ctx->nc == 3)
+        hipLaunchKernelGGL(k_cg_update_p<12>, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, r, p, (int)ctx->V, st);
+    else
+        hipLaunchKernelGGL(k_cg_update_p<8>, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, r, p, (int)ctx->V, st);
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }

@@ -20,7 +20,8 @@ OBJ = os.path.join(HERE, "_build")
 # (source, object name, extra defines): the low-precision stencil is one
 # object per (precision pair, RHS per work item) so those build in parallel.
 SOURCES = [(s, os.path.splitext(s)[0], []) for s in
-           ["wilson.hip", "api.cpp", "blas.hip", "clover.hip", "force.hip", "layout.hip", "mixed.hip", "rng.hip"]]
+           ["wilson.hip", "api.cpp", "blas.hip", "clover.hip", "force.hip", "layout.hip", "mixed.hip", "rng.hip",
+            "wilson_nc.hip"]]
 SOURCES[1:1] = [("wilson_lo.hip", f"wilson_lo_c{c}_nr{n}", [f"-DLMX_LO_COMBO={c}", f"-DLMX_LO_NR={n}"])
                 for c in (0, 1, 2) for n in (1, 2, 4)]
 ARCH = os.environ.get("LMX_OFFLOAD_ARCH", "gfx950")

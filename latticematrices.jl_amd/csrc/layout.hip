@@ -2,8 +2,8 @@
 // arrays (LatticeMatrix.A, src/LatticeMatrices_core.jl:361-363) and the
 // library's structure-of-arrays layout.
 //
-// Reference spinor: A[3,4,L1+2nw,L2+2nw,L3+2nw,L4+2nw] complex128, colour
-// fastest; links: A[3,3,...] per direction, row fastest.  Export refills the
+// Reference spinor: A[NC,4,L1+2nw,L2+2nw,L3+2nw,L4+2nw] complex128, colour
+// fastest; links: A[NC,NC,...] per direction, row fastest (NC the context's).  Export refills the
 // halo exactly as set_halo! would: ghost = core value times the boundary
 // phase (forward wrap) or its inverse (backward wrap), direction by direction
 // (exchange_dim_local!, :707-723), links periodic.
@@ -22,21 +22,20 @@ __device__ __forceinline__ long padded_site(const int x[4], const Pad& p) {
 }
 
 __global__ void __launch_bounds__(256) k_spinor_import(double2* __restrict__ dst, const double2* __restrict__ src,
-                                                       Geom g, Pad p, int nrhs) {
+                                                       Geom g, Pad p, int nrhs, int ncomp) {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= (long)g.V * nrhs) return;
     const int s = (int)(t % g.V);
     const int r = (int)(t / g.V);
     int x[4];
     site_coords(s, g, x);
-    const double2* a = src + (long)r * p.Vp * 12 + padded_site(x, p) * 12;
-    double2* o = dst + (long)r * 12 * g.V;
-#pragma unroll
-    for (int k = 0; k < 12; ++k) o[(long)k * g.V + s] = a[k];
+    const double2* a = src + (long)r * p.Vp * ncomp + padded_site(x, p) * ncomp;
+    double2* o = dst + (long)r * ncomp * g.V;
+    for (int k = 0; k < ncomp; ++k) o[(long)k * g.V + s] = a[k];
 }
 
 __global__ void __launch_bounds__(256) k_spinor_export(double2* __restrict__ dst, const double2* __restrict__ src,
-                                                       Geom g, Pad p, int nrhs) {
+                                                       Geom g, Pad p, int nrhs, int ncomp) {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= p.Vp * nrhs) return;
     const long ps = t % p.Vp;
@@ -61,9 +60,9 @@ __global__ void __launch_bounds__(256) k_spinor_export(double2* __restrict__ dst
         x[d] = c;
     }
     const int s = x[0] + g.stride[1] * x[1] + g.stride[2] * x[2] + g.stride[3] * x[3];
-    const double2* in = src + (long)r * 12 * g.V;
-    double2* o = dst + (long)r * p.Vp * 12 + ps * 12;
-    for (int k = 0; k < 12; ++k) {
+    const double2* in = src + (long)r * ncomp * g.V;
+    double2* o = dst + (long)r * p.Vp * ncomp + ps * ncomp;
+    for (int k = 0; k < ncomp; ++k) {
         double2 v = in[(long)k * g.V + s];
         cx<double> z = mk<double>(v.x, v.y);
         for (int d = 0; d < 4; ++d) {
@@ -82,20 +81,21 @@ struct LinkOutPtrs {
     double2* p[4];
 };
 
-__global__ void __launch_bounds__(256) k_links_import(double2* __restrict__ dst, LinkPtrs src, Geom g, Pad p) {
+__global__ void __launch_bounds__(256) k_links_import(double2* __restrict__ dst, LinkPtrs src, Geom g, Pad p, int nc) {
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= g.V) return;
     int x[4];
     site_coords(s, g, x);
     const long ps = padded_site(x, p);
     for (int mu = 0; mu < 4; ++mu) {
-        const double2* a = src.p[mu] + ps * 9;
-        for (int row = 0; row < 3; ++row)
-            for (int col = 0; col < 3; ++col) dst[(long)(mu * 9 + row * 3 + col) * g.V + s] = a[row + 3 * col];
+        const double2* a = src.p[mu] + ps * nc * nc;
+        for (int row = 0; row < nc; ++row)
+            for (int col = 0; col < nc; ++col) dst[(long)(mu * nc * nc + row * nc + col) * g.V + s] = a[row + nc * col];
     }
 }
 
-__global__ void __launch_bounds__(256) k_links_export(LinkOutPtrs dst, const double2* __restrict__ src, Geom g, Pad p) {
+__global__ void __launch_bounds__(256) k_links_export(LinkOutPtrs dst, const double2* __restrict__ src, Geom g, Pad p,
+                                                      int nc) {
     const long ps = (long)blockIdx.x * 256 + threadIdx.x;
     if (ps >= p.Vp) return;
     int x[4];
@@ -109,9 +109,9 @@ __global__ void __launch_bounds__(256) k_links_export(LinkOutPtrs dst, const dou
     }
     const int s = x[0] + g.stride[1] * x[1] + g.stride[2] * x[2] + g.stride[3] * x[3];
     for (int mu = 0; mu < 4; ++mu) {
-        double2* a = dst.p[mu] + ps * 9;
-        for (int row = 0; row < 3; ++row)
-            for (int col = 0; col < 3; ++col) a[row + 3 * col] = src[(long)(mu * 9 + row * 3 + col) * g.V + s];
+        double2* a = dst.p[mu] + ps * nc * nc;
+        for (int row = 0; row < nc; ++row)
+            for (int col = 0; col < nc; ++col) a[row + nc * col] = src[(long)(mu * nc * nc + row * nc + col) * g.V + s];
     }
 }
 
@@ -140,7 +140,7 @@ int spinor_import(lmx_context* ctx, void* dst, const void* src, int nw, int nrhs
     const long n = ctx->V * (long)nrhs;
     if (n == 0) return LMX_OK;
     hipLaunchKernelGGL(k_spinor_import, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (double2*)dst,
-                       (const double2*)src, ctx->g, p, nrhs);
+                       (const double2*)src, ctx->g, p, nrhs, spinor_ncomp(ctx));
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }
@@ -151,7 +151,7 @@ int spinor_export(lmx_context* ctx, void* dst, int nw, const void* src, int nrhs
     const long n = p.Vp * nrhs;
     if (n == 0) return LMX_OK;
     hipLaunchKernelGGL(k_spinor_export, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (double2*)dst,
-                       (const double2*)src, ctx->g, p, nrhs);
+                       (const double2*)src, ctx->g, p, nrhs, spinor_ncomp(ctx));
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }
@@ -165,7 +165,8 @@ int links_import(lmx_context* ctx, void* dst, const void* const src[4], int nw) 
         lp.p[mu] = (const double2*)src[mu];
     }
     if (ctx->V == 0) return LMX_OK;
-    hipLaunchKernelGGL(k_links_import, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, (double2*)dst, lp, ctx->g, p);
+    hipLaunchKernelGGL(k_links_import, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, (double2*)dst, lp, ctx->g, p,
+                       ctx->nc);
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }
@@ -180,7 +181,7 @@ int links_export(lmx_context* ctx, void* const dst[4], int nw, const void* src) 
     }
     if (p.Vp == 0) return LMX_OK;
     hipLaunchKernelGGL(k_links_export, dim3((unsigned)((p.Vp + 255) / 256)), dim3(256), 0, ctx->stream, lp,
-                       (const double2*)src, ctx->g, p);
+                       (const double2*)src, ctx->g, p, ctx->nc);
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }

@@ -55,6 +55,7 @@ struct CGState {
 struct lmx_context {
     lmx::Geom g;
     int64_t V;
+    int nc;  // colours of the fields (3: the half-spin kernels; 2: the generic-NC kernels)
     hipStream_t stream;
     int device;
     // Reduction scratch: per-block partials and self-resetting completion counters.
@@ -167,6 +168,18 @@ int launch_clover_build_f64(lmx_context* ctx, const double2* U, double2* clover)
 int launch_clover_export(lmx_context* ctx, const double2* clover, double2* F_out);
 int launch_convert_links(lmx_context* ctx, const double2* U, float2* U32, __half2* U16);
 int launch_convert_clover(lmx_context* ctx, const double2* F, float2* F32, __half2* F16);
+
+// ---- generic-NC FP64 kernels (wilson_nc.hip; ctx->nc != 3) ----
+// nw < 0: library SoA spinors; nw >= 0: the reference's padded storage.
+int launch_wilson_nc(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
+                     double2* out, const double2* in, int nrhs, int flags, int nw, const CGState* st,
+                     const int* gate);
+int launch_clover_build_nc(lmx_context* ctx, const double2* U, double2* F);
+int launch_clover_export_nc(lmx_context* ctx, const double2* F, double2* out);
+int launch_pullback_nc(lmx_context* ctx, double2* dU, const double2* dres, const double2* psi, double coefficient);
+// complex components per spinor site and entries per link
+inline int spinor_ncomp(const lmx_context* ctx) { return 4 * ctx->nc; }
+inline int link_nelem(const lmx_context* ctx) { return ctx->nc * ctx->nc; }
 
 // ---- BLAS / CG kernels (blas.hip) ----
 enum { FIN_OUT = 0, FIN_BB = 1, FIN_RR0 = 2, FIN_RRNEW = 3, FIN_PAP = 4 };

@@ -757,25 +757,21 @@ static bool cache_lookup(const char* path, const lmx_context* c, int prec, int n
 
 }  // namespace lmx
 
-extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char* cache_path, lmx_tune_result* out) {
-    using namespace lmx;
-    clear_error();
-    LMX_CHECK(op != nullptr && op->ctx != nullptr && out != nullptr, LMX_ERR_ARG, "bad autotune arguments");
-    LMX_CHECK(precision == LMX_F32 || precision == LMX_F16 || precision == LMX_H16, LMX_ERR_ARG,
-              "autotune precision must be F32, F16 or H16");
-    LMX_CHECK(nrhs >= 1 && nrhs <= 64, LMX_ERR_ARG, "nrhs must be in [1, 64]");
-    lmx_context* c = op->ctx;
-    const std::string arch = arch_name(c->device);
-    int best = 0;
-    double best_us = 0.0;
-    const int clover = op->csw != 0.0 ? 1 : 0;
-    if (cache_lookup(cache_path, c, precision, nrhs, clover, arch, &best, &best_us)) {
-        op->tuned_block[precision][nrhs] = best;
-        out->rhs_block = best;
-        out->block_size = 256;
-        out->ms_per_apply = best_us * 1e-3;
-        return LMX_OK;
+namespace lmx {
+// The buffers and launch arguments of one group of the mixed solver's inner
+// iterations (D with |D p|^2, then D^dagger with the r update and the x update
+// deferred or applied), for timing launch shapes (lmx_autotune) and single
+// kernels (lmx_inner_kernel_times).  Freed by the destructor.
+struct InnerGroup {
+    char* buf = nullptr;
+    LoArgs ld, lx;
+    ~InnerGroup() {
+        if (buf) (void)hipFree(buf);
     }
+};
+
+static int inner_group_setup(lmx_wilson* op, int precision, int nrhs, InnerGroup& G) {
+    lmx_context* c = op->ctx;
     LMX_TRY(ensure_lo_links(op));
     const int sp = spinor_prec_of(precision);
     // buffers of one group of inner iterations: p, t = D p, r, x_inner,
@@ -787,12 +783,13 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     const size_t pb = align256(2 * nbk * nrhs * sizeof(double));
     const size_t ab = align256((nrhs + kXBatch) * sizeof(double)), db = align256(nrhs * sizeof(int));
     const size_t total = 3 * fb + xb + pb + ab + 2 * db + (kXBatch - 1) * fb;
-    char* buf = nullptr;
-    if (hipMalloc(&buf, total) != hipSuccess) {
-        set_error("autotune buffer allocation of %zu bytes failed", total);
+    if (hipMalloc(&G.buf, total) != hipSuccess) {
+        G.buf = nullptr;
+        set_error("inner-iteration buffer allocation of %zu bytes failed", total);
         return LMX_ERR_NOMEM;
     }
-    (void)hipMemsetAsync(buf, 0, total, c->stream);
+    char* buf = G.buf;
+    LMX_HIP(hipMemsetAsync(buf, 0, total, c->stream));
     char* pbuf = buf;
     char* tbuf = buf + fb;
     char* rbuf = buf + 2 * fb;
@@ -803,14 +800,12 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     int* masks = (int*)((char*)done + db);
     char* qbuf = (char*)masks + db;  // kXBatch - 1 deferred p slots
     static_assert(kXBatch >= 2 && kXBatch <= 8, "deferred-slot masks are filled bytewise");
-    (void)hipMemsetAsync(masks, (1 << (kXBatch - 1)) - 1, db, c->stream);  // slot bits 0..kXBatch-2 set
+    LMX_HIP(hipMemsetAsync(masks, (1 << (kXBatch - 1)) - 1, db, c->stream));  // slot bits 0..kXBatch-2 set
     auto norm_of = [&](char* q) -> float* {  // per-site scales follow the 16-bit data
         return sp == LMX_H16 ? (float*)(q + (size_t)nrhs * c->V * 3 * sizeof(int4)) : nullptr;
     };
-    hipEvent_t e0, e1;
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    LoArgs ld = lo_args(op, link_prec_of(precision), sp);  // t = D p, |t|^2
+    LoArgs& ld = G.ld;  // t = D p, |t|^2
+    ld = lo_args(op, link_prec_of(precision), sp);
     ld.nrhs = nrhs;
     ld.in = pbuf;
     ld.in_norm = norm_of(pbuf);
@@ -819,7 +814,8 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     ld.epi = 2;
     ld.partials = parts;
     ld.pstride = 2L * (long)nbk;
-    LoArgs lx = lo_args(op, link_prec_of(precision), sp);  // D^dagger t with the x / r update
+    LoArgs& lx = G.lx;  // D^dagger t with the x / r update
+    lx = lo_args(op, link_prec_of(precision), sp);
     lx.nrhs = nrhs;
     lx.in = tbuf;
     lx.in_norm = norm_of(tbuf);
@@ -842,6 +838,38 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     }
     lx.partials = parts;
     lx.pstride = 2L * (long)nbk;
+    return LMX_OK;
+}
+}  // namespace lmx
+
+extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char* cache_path, lmx_tune_result* out) {
+    using namespace lmx;
+    clear_error();
+    LMX_CHECK(op != nullptr && op->ctx != nullptr && out != nullptr, LMX_ERR_ARG, "bad autotune arguments");
+    LMX_CHECK(precision == LMX_F32 || precision == LMX_F16 || precision == LMX_H16, LMX_ERR_ARG,
+              "autotune precision must be F32, F16 or H16");
+    LMX_CHECK(nrhs >= 1 && nrhs <= 64, LMX_ERR_ARG, "nrhs must be in [1, 64]");
+    lmx_context* c = op->ctx;
+    LMX_CHECK(c->nc == 3, LMX_ERR_UNSUPPORTED, "the autotuner is built for NC = 3 fields (this context has NC = %d)",
+              c->nc);
+    const std::string arch = arch_name(c->device);
+    int best = 0;
+    double best_us = 0.0;
+    const int clover = op->csw != 0.0 ? 1 : 0;
+    if (cache_lookup(cache_path, c, precision, nrhs, clover, arch, &best, &best_us)) {
+        op->tuned_block[precision][nrhs] = best;
+        out->rhs_block = best;
+        out->block_size = 256;
+        out->ms_per_apply = best_us * 1e-3;
+        return LMX_OK;
+    }
+    InnerGroup G;
+    LMX_TRY(inner_group_setup(op, precision, nrhs, G));
+    LoArgs& ld = G.ld;
+    LoArgs& lx = G.lx;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
     auto group = [&]() {
         int rc2 = LMX_OK;
         for (int j = 0; j < kXBatch && rc2 == LMX_OK; ++j) {
@@ -893,7 +921,6 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(buf);
     LMX_TRY(rc);
     op->tuned_block[precision][nrhs] = best;
     if (cache_path) {
@@ -907,5 +934,51 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
     out->rhs_block = best;
     out->block_size = 256;
     out->ms_per_apply = best_us * 1e-3;
+    return LMX_OK;
+}
+
+extern "C" int lmx_inner_kernel_times(lmx_wilson* op, int precision, int nrhs, int rhs_block, int groups,
+                                      double us[2]) {
+    using namespace lmx;
+    clear_error();
+    LMX_CHECK(op != nullptr && op->ctx != nullptr && us != nullptr, LMX_ERR_ARG, "bad inner_kernel_times arguments");
+    LMX_CHECK(precision == LMX_F32 || precision == LMX_F16 || precision == LMX_H16, LMX_ERR_ARG,
+              "precision must be F32, F16 or H16");
+    LMX_CHECK(nrhs >= 1 && nrhs <= 64 && groups >= 1, LMX_ERR_ARG, "nrhs must be in [1, 64], groups positive");
+    lmx_context* c = op->ctx;
+    LMX_CHECK(c->nc == 3, LMX_ERR_UNSUPPORTED, "the mixed solver's kernels are built for NC = 3 fields");
+    if (rhs_block == 0) rhs_block = op->tuned_block[precision][nrhs];
+    InnerGroup G;
+    LMX_TRY(inner_group_setup(op, precision, nrhs, G));
+    G.ld.rhs_block = G.lx.rhs_block = rhs_block;
+    // one event before the first launch and one after each launch: kernel k's
+    // duration is the interval that ends at its own event
+    const int nl = 2 * kXBatch * (groups + 1);
+    std::vector<hipEvent_t> ev(nl + 1);
+    for (auto& e : ev) LMX_HIP(hipEventCreate(&e));
+    int rc = LMX_OK;
+    int k = 0;
+    LMX_HIP(hipEventRecord(ev[0], c->stream));
+    for (int g = 0; g <= groups && rc == LMX_OK; ++g) {  // group 0 warms up
+        for (int j = 0; j < kXBatch && rc == LMX_OK; ++j) {
+            G.lx.apply_x = j == kXBatch - 1 ? 1 : 0;
+            rc = launch_wilson_lo(c, G.ld);
+            if (hipEventRecord(ev[++k], c->stream) != hipSuccess) rc = LMX_ERR_HIP;
+            if (rc == LMX_OK) rc = launch_wilson_lo(c, G.lx);
+            if (rc == LMX_OK && hipEventRecord(ev[++k], c->stream) != hipSuccess) rc = LMX_ERR_HIP;
+        }
+    }
+    if (rc == LMX_OK && hipEventSynchronize(ev[nl]) != hipSuccess) rc = LMX_ERR_HIP;
+    double tot[2] = {0.0, 0.0};
+    for (int i = 2 * kXBatch; i < nl && rc == LMX_OK; ++i) {  // skip the warm-up group
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) rc = LMX_ERR_HIP;
+        tot[i % 2] += ms;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    LMX_TRY(rc);
+    const double per = (double)groups * kXBatch;
+    us[0] = 1e3 * tot[0] / per;  // D with |D p|^2
+    us[1] = 1e3 * tot[1] / per;  // D^dagger with the r update, x deferred 3 of 4 times
     return LMX_OK;
 }

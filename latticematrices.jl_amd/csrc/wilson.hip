@@ -12,6 +12,10 @@ LMX_LO_ALL(LMX_EXTERN)
 int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
                       double2* out, const double2* in, int nrhs, int flags, const Epilogue& e) {
     if (ctx->V == 0 || nrhs == 0) return LMX_OK;
+    if (ctx->nc != 3) {  // the reference's generic-NC kernels (wilson_nc.hip)
+        LMX_CHECK(e.kind == 0, LMX_ERR_UNSUPPORTED, "fused epilogues are built for NC = 3");
+        return launch_wilson_nc(ctx, U, clover, kappa, csw, out, in, nrhs, flags, -1, e.st, e.gate);
+    }
     const double ccoef = (-kappa) * csw;  // coefficient = -kappa * cSW (:572)
     const int V = (int)ctx->V;
     SpF64Out o{out, V};

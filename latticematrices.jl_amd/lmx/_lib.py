@@ -70,6 +70,7 @@ SIGNATURES = {
     "lmx_last_error": (ctypes.c_char_p, []),
     "lmx_abi_version": (I, []),
     "lmx_context_create": (I, [ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int32), P, P]),
+    "lmx_context_create_nc": (I, [ctypes.POINTER(P), ctypes.POINTER(ctypes.c_int32), P, P, I]),
     "lmx_context_destroy": (I, [P]),
     "lmx_context_set_stream": (I, [P, P]),
     "lmx_context_synchronize": (I, [P]),
@@ -97,6 +98,7 @@ SIGNATURES = {
     "lmx_cg_params_default": (None, [ctypes.POINTER(CGParams)]),
     "lmx_cg_ddagd": (I, [P, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
     "lmx_cg_ddagd_noalloc": (I, [P, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
+    "lmx_cg_ddagd_padded": (I, [P, P, P, I, P, ctypes.c_size_t, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
     "lmx_cg_generic": (I, [P, APPLY_FN, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
     "lmx_cg_batched_workspace": (I, [P, I, ctypes.POINTER(CGParams), ctypes.POINTER(ctypes.c_size_t)]),
     "lmx_cg_batched": (I, [P, P, P, I, P, ctypes.c_size_t, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
@@ -104,6 +106,7 @@ SIGNATURES = {
     "lmx_ta_project": (I, [P, P, P, P]),
     "lmx_wilson_force": (I, [P, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
     "lmx_autotune": (I, [P, I, I, ctypes.c_char_p, ctypes.POINTER(TuneResult)]),
+    "lmx_inner_kernel_times": (I, [P, I, I, I, I, ctypes.POINTER(D)]),
     "lmx_field_pad": (I, [P, P, P, I, I, I, I]),
     "lmx_field_gather": (I, [P, P, P, I, I, I, I]),
     "lmx_site_rng_draw": (I, [P, ctypes.POINTER(RNGKey), I, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(RNGDraws),

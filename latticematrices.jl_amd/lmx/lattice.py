@@ -50,9 +50,11 @@ def _ptr(t) -> int:
 
 class Lattice:
     """Geometry, spinor boundary phases and the HIP stream of one lattice
-    (the single-rank LatticeMatrix construction, src/LatticeMatrices_core.jl:297-343)."""
+    (the single-rank LatticeMatrix construction, src/LatticeMatrices_core.jl:297-343).
+    nc: colours of its fields (NC1 of LatticeMatrix; 3, or 2 for the
+    reference's generic-NC kernels)."""
 
-    def __init__(self, dims, phases=(1, 1, 1, 1), device=None):
+    def __init__(self, dims, phases=(1, 1, 1, 1), device=None, nc: int = 3):
         _require_gpu()
         dims = tuple(int(d) for d in dims)
         if len(dims) != 4:
@@ -63,12 +65,15 @@ class Lattice:
         self.phases = tuple(complex(p) for p in phases)
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         self.V = int(np.prod(dims))
+        self.nc = int(nc)
+        self.ncomp = 4 * self.nc  # complex spinor components per site
         with torch.cuda.device(self.device):
             self.stream = torch.cuda.current_stream(self.device)
         ph = (ctypes.c_double * 8)(*[v for p in self.phases for v in (p.real, p.imag)])
         d = (ctypes.c_int32 * 4)(*dims)
         h = ctypes.c_void_p()
-        check(lib().lmx_context_create(ctypes.byref(h), d, ph, ctypes.c_void_p(self.stream.cuda_stream)))
+        check(lib().lmx_context_create_nc(ctypes.byref(h), d, ph, ctypes.c_void_p(self.stream.cuda_stream),
+                                          self.nc))
         self._ctx = h
 
     @property
@@ -98,14 +103,14 @@ class Lattice:
 
 
 class SpinorField:
-    """nrhs Dirac spinor fields (3 colours x 4 spins per site), FP64, in the
-    device layout double2[nrhs][12][V]."""
+    """nrhs Dirac spinor fields (nc colours x 4 spins per site), FP64, in the
+    device layout double2[nrhs][4 nc][V]."""
 
     def __init__(self, lattice: Lattice, nrhs: int = 1, data=None):
         self.lattice = lattice
         self.nrhs = int(nrhs)
         if data is None:
-            data = torch.zeros((self.nrhs, 12, lattice.V, 2), dtype=torch.float64, device=lattice.device)
+            data = torch.zeros((self.nrhs, lattice.ncomp, lattice.V, 2), dtype=torch.float64, device=lattice.device)
         self.data = data
 
     @property
@@ -117,10 +122,10 @@ class SpinorField:
 
     @classmethod
     def from_numpy(cls, lattice: Lattice, arr, nw: int = 0) -> "SpinorField":
-        """arr: complex [V,4,3] / [nrhs,V,4,3] (nw = 0) or the padded reference
-        array(s) [nrhs, Lt+2nw, Lz+2nw, Ly+2nw, Lx+2nw, 4, 3] in C order."""
+        """arr: complex [V,4,nc] / [nrhs,V,4,nc] (nw = 0) or the padded reference
+        array(s) [nrhs, Lt+2nw, Lz+2nw, Ly+2nw, Lx+2nw, 4, nc] in C order."""
         a = np.ascontiguousarray(arr, dtype=np.complex128)
-        per = int(np.prod([d + 2 * nw for d in lattice.dims])) * 12
+        per = int(np.prod([d + 2 * nw for d in lattice.dims])) * lattice.ncomp
         if a.size % per != 0:
             raise ArgumentError(_lib.LMX_ERR_ARG, f"array of {a.size} elements is not a multiple of {per}")
         nrhs = a.size // per
@@ -132,15 +137,15 @@ class SpinorField:
 
     def to_padded(self, nw: int = 0):
         L = self.lattice
-        n = self.nrhs * int(np.prod([d + 2 * nw for d in L.dims])) * 12
+        n = self.nrhs * int(np.prod([d + 2 * nw for d in L.dims])) * L.ncomp
         dst = torch.empty(n, dtype=torch.complex128, device=L.device)
         check(lib().lmx_spinor_export(L.ctx, ctypes.c_void_p(_ptr(dst)), int(nw), self.ptr, self.nrhs))
         L.synchronize()
         return dst.cpu().numpy()
 
     def to_numpy(self):
-        """[V,4,3] (nrhs == 1) or [nrhs,V,4,3] complex128."""
-        a = self.to_padded(0).reshape(self.nrhs, self.lattice.V, 4, 3)
+        """[V,4,nc] (nrhs == 1) or [nrhs,V,4,nc] complex128."""
+        a = self.to_padded(0).reshape(self.nrhs, self.lattice.V, 4, self.lattice.nc)
         return a[0] if self.nrhs == 1 else a
 
     def rhs(self, i: int) -> "SpinorField":
@@ -149,12 +154,12 @@ class SpinorField:
 
 
 class GaugeField:
-    """The four link fields U_1..U_4 (SU(3)), FP64, device layout double2[4][9][V]."""
+    """The four link fields U_1..U_4 (nc x nc each), FP64, device layout double2[4][nc nc][V]."""
 
     def __init__(self, lattice: Lattice, data=None):
         self.lattice = lattice
         if data is None:
-            data = torch.zeros((4, 9, lattice.V, 2), dtype=torch.float64, device=lattice.device)
+            data = torch.zeros((4, lattice.nc * lattice.nc, lattice.V, 2), dtype=torch.float64, device=lattice.device)
         self.data = data
         self._core_epoch = 0
 
@@ -179,12 +184,13 @@ class GaugeField:
 
     @classmethod
     def from_numpy(cls, lattice: Lattice, U, nw: int = 0) -> "GaugeField":
-        """U: complex [4, V, 3, 3] (mu, site, row, col) with nw = 0, or four padded
-        reference link arrays (any container of 4 arrays of V_padded*9 values in
-        the reference's column-major element order when nw > 0)."""
+        """U: complex [4, V, nc, nc] (mu, site, row, col) with nw = 0, or four padded
+        reference link arrays (any container of 4 arrays of V_padded*nc*nc values
+        in the reference's column-major element order when nw > 0)."""
         g = cls(lattice)
+        nc = lattice.nc
         if nw == 0 and not isinstance(U, (list, tuple)):
-            a = np.ascontiguousarray(U, dtype=np.complex128).reshape(4, lattice.V, 3, 3)
+            a = np.ascontiguousarray(U, dtype=np.complex128).reshape(4, lattice.V, nc, nc)
             # reference element order within a site is column-major (row fastest)
             arrs = [np.ascontiguousarray(np.swapaxes(a[mu], -1, -2)).reshape(-1) for mu in range(4)]
         else:
@@ -196,13 +202,14 @@ class GaugeField:
         return g
 
     def to_numpy(self):
-        """[4, V, 3, 3] complex128 (mu, site, row, col)."""
+        """[4, V, nc, nc] complex128 (mu, site, row, col)."""
         L = self.lattice
-        dev = [torch.empty(L.V * 9, dtype=torch.complex128, device=L.device) for _ in range(4)]
+        nc = L.nc
+        dev = [torch.empty(L.V * nc * nc, dtype=torch.complex128, device=L.device) for _ in range(4)]
         ptrs = (ctypes.c_void_p * 4)(*[_ptr(t) for t in dev])
         check(lib().lmx_links_export(L.ctx, ptrs, 0, self.ptr))
         L.synchronize()
-        out = np.stack([t.cpu().numpy().reshape(L.V, 3, 3) for t in dev])
+        out = np.stack([t.cpu().numpy().reshape(L.V, nc, nc) for t in dev])
         return np.ascontiguousarray(np.swapaxes(out, -1, -2))
 
 
@@ -248,13 +255,13 @@ class WilsonDiracOperator4D:
         self._source = (self.U, self.U.core_epoch)
 
     def clover_planes(self):
-        """The six clover field-strength planes [6, V, 3, 3] (CloverFieldStrength4D)."""
+        """The six clover field-strength planes [6, V, nc, nc] (CloverFieldStrength4D)."""
         return self.clover_planes_device().cpu().numpy()
 
     def clover_planes_device(self):
-        """The same planes as a complex128 device tensor [6, V, 3, 3] (plane, site, row, col)."""
+        """The same planes as a complex128 device tensor [6, V, nc, nc] (plane, site, row, col)."""
         L = self.lattice
-        out = torch.empty((6, L.V, 3, 3), dtype=torch.complex128, device=L.device)
+        out = torch.empty((6, L.V, L.nc, L.nc), dtype=torch.complex128, device=L.device)
         check(lib().lmx_clover_export(self._op, ctypes.c_void_p(_ptr(out))))
         L.synchronize()
         return out
@@ -576,13 +583,13 @@ def mul_padded_(C, A, psi, nw: int):
     """mul!(C, A, psi) on arrays in the reference's own storage (no
     conversion): C and psi are device complex128 (ComplexF64) or complex64
     (ComplexF32) tensors laid out as LatticeMatrix.A, i.e. C-order [nrhs?,
-    Lt+2nw, Lz+2nw, Ly+2nw, Lx+2nw, 4, 3] (column-major [3,4,Lx+2nw,...]).
+    Lt+2nw, Lz+2nw, Ly+2nw, Lx+2nw, 4, nc] (column-major [nc,4,Lx+2nw,...]).
     psi's halo is ignored (in-kernel wrap with the lattice's phases); only C's
     core sites are written.  A is a Wilson(-clover) operator, its adjoint, or
     the hopping-only operator; complex64 fields use FP32 links and arithmetic."""
     h, flags = _wilson_handle(A)
     lat = (A.parent if isinstance(A, Adjoint) else A).lattice
-    per = int(np.prod([d + 2 * nw for d in lat.dims])) * 12
+    per = int(np.prod([d + 2 * nw for d in lat.dims])) * lat.ncomp
     dt = psi.dtype if isinstance(psi, torch.Tensor) else None
     for t in (C, psi):
         if not (isinstance(t, torch.Tensor) and t.dtype in (torch.complex128, torch.complex64) and t.dtype == dt
@@ -769,7 +776,7 @@ def dSFdU(dfdU: GaugeField, D: WilsonDiracOperator4D, phi: SpinorField, force: G
     `force`, also the su(3) force TA(U dfdU^dagger).  Returns (CGResult, eta, chi)."""
     L = D.lattice
     eta, chi = SpinorField(L), SpinorField(L)
-    work = torch.empty((4, 12, L.V, 2), dtype=torch.float64, device=L.device)
+    work = torch.empty((4, L.ncomp, L.V, 2), dtype=torch.float64, device=L.device)
     params = _lib.default_params(rtol=float(rtol), maxiter=int(maxiter))
     res = _lib.CGResultC()
     check(lib().lmx_wilson_force(D._op, dfdU.ptr, force.ptr if force is not None else None, phi.ptr, eta.ptr,
@@ -830,6 +837,16 @@ def cg_batched_(x: SpinorField, D: WilsonDiracOperator4D, b: SpinorField, precis
     check(lib().lmx_cg_batched(D._op, x.ptr, b.ptr, x.nrhs, ctypes.c_void_p(workspace.data_ptr()),
                                nbytes.value, ctypes.byref(params), res))
     return [CGResult.from_c(r) for r in res]
+
+
+def inner_kernel_times(D: WilsonDiracOperator4D, precision: str, nrhs: int, rhs_block: int = 0, groups: int = 4):
+    """Microseconds per launch of the mixed solver's two inner stencils at a
+    launch shape (0: the autotuned one): {"D": D with |Dp|^2, "Ddag": D^dagger
+    with the r / x update}, HIP events on the library stream."""
+    _refresh_low_precision(D)
+    us = (ctypes.c_double * 2)()
+    check(lib().lmx_inner_kernel_times(D._op, _PREC[precision], int(nrhs), int(rhs_block), int(groups), us))
+    return {"D": us[0], "Ddag": us[1]}
 
 
 def autotune(D: WilsonDiracOperator4D, precision: str, nrhs: int, cache_path: str | None = None):

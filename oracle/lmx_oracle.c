@@ -279,11 +279,12 @@ void orc_wilson(double* out, const double* in, const double* U, const int* L, co
 
 /* kernel_dot_D per-site value, src/LinearAlgebras/linearalgebra_D.jl:3331-3341:
  * s = 0; for spin, for colour: s += conj(A)*B */
-static inline cx site_dot(const double* a, const double* b, long s) {
+static inline cx site_dot(const double* a, const double* b, long s, int ncomp) {
     cx acc = C(0.0, 0.0);
-    const double* pa = a + s * 24;
-    const double* pb = b + s * 24;
-    for (int k = 0; k < 12; ++k) acc = cadd(acc, cmul(cconj(C(pa[2 * k], pa[2 * k + 1])), C(pb[2 * k], pb[2 * k + 1])));
+    const double* pa = a + s * 2 * ncomp;
+    const double* pb = b + s * 2 * ncomp;
+    for (int k = 0; k < ncomp; ++k)
+        acc = cadd(acc, cmul(cconj(C(pa[2 * k], pa[2 * k + 1])), C(pb[2 * k], pb[2 * k + 1])));
     return acc;
 }
 
@@ -350,27 +351,30 @@ static cx reduce_sites(long V, int order, site_fn f, const void* ctx) {
 
 typedef struct {
     const double *a, *b;
+    int ncomp;
 } dot_ctx;
 static cx dot_site_fn(const void* c, long s) {
     const dot_ctx* d = (const dot_ctx*)c;
-    return site_dot(d->a, d->b, s);
+    return site_dot(d->a, d->b, s, d->ncomp);
 }
 
-/* LinearAlgebra.dot, src/LinearAlgebras/linearalgebra_D.jl:3325-3329 (one rank) */
-void orc_dot(const double* a, const double* b, long V, int order, double* out2) {
-    dot_ctx d = {a, b};
+/* LinearAlgebra.dot, src/LinearAlgebras/linearalgebra_D.jl:3325-3329 (one
+ * rank), over sites of ncomp = 4 NC complex components each */
+void orc_dot_n(const double* a, const double* b, long V, int ncomp, int order, double* out2) {
+    dot_ctx d = {a, b, ncomp};
     cx z = reduce_sites(V, order, dot_site_fn, &d);
     out2[0] = z.re;
     out2[1] = z.im;
 }
+void orc_dot(const double* a, const double* b, long V, int order, double* out2) { orc_dot_n(a, b, V, 12, order, out2); }
 
 /* axpby!(a, X, b, Y): Y = a*X + b*Y, src/LinearAlgebras/linearalgebra_D.jl:2-23.
  * a and b are real here (CG only passes reals). */
-void orc_axpby(double a, const double* X, double b, double* Y, long V) {
-    long n = V * 24;
+static void axpby_n(double a, const double* X, double b, double* Y, long n) {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) Y[i] = a * X[i] + b * Y[i];
 }
+void orc_axpby(double a, const double* X, double b, double* Y, long V) { axpby_n(a, X, b, Y, V * 24); }
 
 /* ------------------------------------------------------------------------ */
 /* Clover                                                                   */
@@ -554,11 +558,19 @@ typedef struct {
     double kappa, csw;
     double scale_re, scale_im;
     double* tmp; /* D x temporary (DdagDOp temp) */
+    int nc;      /* colours (0 means 3); NC != 3 runs the generic kernels of lmx_nc_oracle.c */
 } orc_op;
 
-static void scale_field(double* out, const double* in, cx a, long V) {
+void orc_wilson_nc(double* out, const double* in, const double* U, const int* L, const double* phases, double kappa,
+                   int nc, int dagger, int donly);
+void orc_wilson_clover_nc(double* out, const double* in, const double* U, const double* F, const int* L,
+                          const double* phases, double kappa, double csw, int nc, int dagger);
+
+static int op_nc(const orc_op* op) { return op->nc > 0 ? op->nc : 3; }
+
+static void scale_field(double* out, const double* in, cx a, long n) {
 #pragma omp parallel for schedule(static)
-    for (long i = 0; i < V * 12; ++i) {
+    for (long i = 0; i < n; ++i) {
         cx v = cmul(a, C(in[2 * i], in[2 * i + 1]));
         out[2 * i] = v.re;
         out[2 * i + 1] = v.im;
@@ -567,21 +579,33 @@ static void scale_field(double* out, const double* in, cx a, long V) {
 
 /* DdagDOp.mul!: temp = D x; y = D' temp (src/Operators/DiracOperators.jl:104-121) */
 static void op_apply(const orc_op* op, double* y, const double* x, long V) {
+    const int nc = op_nc(op);
+    const long n = V * 4 * nc;
     switch (op->kind) {
     case ORC_OP_DDAGD_WILSON:
-        orc_wilson(op->tmp, x, op->U, op->L, op->phases, op->kappa, 0, 0);
-        orc_wilson(y, op->tmp, op->U, op->L, op->phases, op->kappa, 1, 0);
+        if (nc == 3) {
+            orc_wilson(op->tmp, x, op->U, op->L, op->phases, op->kappa, 0, 0);
+            orc_wilson(y, op->tmp, op->U, op->L, op->phases, op->kappa, 1, 0);
+        } else {
+            orc_wilson_nc(op->tmp, x, op->U, op->L, op->phases, op->kappa, nc, 0, 0);
+            orc_wilson_nc(y, op->tmp, op->U, op->L, op->phases, op->kappa, nc, 1, 0);
+        }
         break;
     case ORC_OP_DDAGD_CLOVER:
-        orc_wilson_clover(op->tmp, x, op->U, op->F, op->L, op->phases, op->kappa, op->csw, 0);
-        orc_wilson_clover(y, op->tmp, op->U, op->F, op->L, op->phases, op->kappa, op->csw, 1);
+        if (nc == 3) {
+            orc_wilson_clover(op->tmp, x, op->U, op->F, op->L, op->phases, op->kappa, op->csw, 0);
+            orc_wilson_clover(y, op->tmp, op->U, op->F, op->L, op->phases, op->kappa, op->csw, 1);
+        } else {
+            orc_wilson_clover_nc(op->tmp, x, op->U, op->F, op->L, op->phases, op->kappa, op->csw, nc, 0);
+            orc_wilson_clover_nc(y, op->tmp, op->U, op->F, op->L, op->phases, op->kappa, op->csw, nc, 1);
+        }
         break;
     case ORC_OP_SCALE: /* mul!(y, scale, x), src/LinearAlgebras/mul.jl:2-9 */
-        scale_field(y, x, C(op->scale_re, op->scale_im), V);
+        scale_field(y, x, C(op->scale_re, op->scale_im), n);
         break;
     default: /* D = scale, D' = conj(scale) */
-        scale_field(op->tmp, x, C(op->scale_re, op->scale_im), V);
-        scale_field(y, op->tmp, C(op->scale_re, -op->scale_im), V);
+        scale_field(op->tmp, x, C(op->scale_re, op->scale_im), n);
+        scale_field(y, op->tmp, C(op->scale_re, -op->scale_im), n);
         break;
     }
 }
@@ -609,8 +633,10 @@ static void set_result(orc_cg_result* r, int conv, int it, double rn, double bn,
 /* cg!, src/LinearAlgebras/cg.jl:73-137, in its operation order. */
 int orc_cg(double* x, const double* b, double* r, double* p, double* Ap, const orc_op* op, long V, double rtol,
            double atol, int maxiter, int dot_order, orc_cg_result* res) {
+    const int ncomp = 4 * op_nc(op);
+    const long n = V * 2 * ncomp; /* doubles per field */
     double d2[2];
-    orc_dot(b, b, V, dot_order, d2);
+    orc_dot_n(b, b, V, ncomp, dot_order, d2);
     double bb = d2[0];
     if (!isfinite(bb) || bb < 0.0) {
         set_result(res, 0, 0, NAN, NAN, ORC_BREAKDOWN);
@@ -619,10 +645,10 @@ int orc_cg(double* x, const double* b, double* r, double* p, double* Ap, const o
     double bnorm = sqrt(bb);
     double tol = fmax(atol, rtol * bnorm);
     op_apply(op, Ap, x, V);
-    memcpy(r, b, sizeof(double) * 24 * V);  /* axpby!(1, b, 0, r) */
-    orc_axpby(-1.0, Ap, 1.0, r, V);         /* axpby!(-1, Ap, 1, r) */
-    memcpy(p, r, sizeof(double) * 24 * V);  /* axpby!(1, r, 0, p) */
-    orc_dot(r, r, V, dot_order, d2);
+    memcpy(r, b, sizeof(double) * n);     /* axpby!(1, b, 0, r) */
+    axpby_n(-1.0, Ap, 1.0, r, n);           /* axpby!(-1, Ap, 1, r) */
+    memcpy(p, r, sizeof(double) * n);     /* axpby!(1, r, 0, p) */
+    orc_dot_n(r, r, V, ncomp, dot_order, d2);
     double rr = d2[0];
     if (!isfinite(rr) || rr < 0.0) {
         set_result(res, 0, 0, NAN, bnorm, ORC_BREAKDOWN);
@@ -635,7 +661,7 @@ int orc_cg(double* x, const double* b, double* r, double* p, double* Ap, const o
     }
     for (int it = 1; it <= maxiter; ++it) {
         op_apply(op, Ap, p, V);
-        orc_dot(p, Ap, V, dot_order, d2);
+        orc_dot_n(p, Ap, V, ncomp, dot_order, d2);
         double pAp = d2[0];
         if (!isfinite(pAp)) {
             set_result(res, 0, it - 1, rn, bnorm, ORC_BREAKDOWN);
@@ -645,9 +671,9 @@ int orc_cg(double* x, const double* b, double* r, double* p, double* Ap, const o
             return 0;
         }
         double alpha = rr / pAp;
-        orc_axpby(alpha, p, 1.0, x, V);
-        orc_axpby(-alpha, Ap, 1.0, r, V);
-        orc_dot(r, r, V, dot_order, d2);
+        axpby_n(alpha, p, 1.0, x, n);
+        axpby_n(-alpha, Ap, 1.0, r, n);
+        orc_dot_n(r, r, V, ncomp, dot_order, d2);
         double rr_new = d2[0];
         if (!isfinite(rr_new) || rr_new < 0.0) {
             set_result(res, 0, it, NAN, bnorm, ORC_BREAKDOWN);
@@ -659,7 +685,7 @@ int orc_cg(double* x, const double* b, double* r, double* p, double* Ap, const o
             return 0;
         }
         double beta = rr_new / rr;
-        orc_axpby(1.0, r, beta, p, V);
+        axpby_n(1.0, r, beta, p, n);
         rr = rr_new;
     }
     set_result(res, 0, maxiter, rn, bnorm, ORC_MAXITER);
@@ -1029,6 +1055,218 @@ int orc_dsfdu_clover(double* G, double* eta, double* chi, const double* U, const
     memset(G, 0, sizeof(double) * 4 * V * 18);
     orc_clover_pullback(G, U, dres, eta, L, phases, -kappa, csw);
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The reference's CPU path of the D^dagger D CG, as cg! runs it on the      */
+/* JACC threads backend: halo-padded AoS fields (LatticeMatrix.A, nw = 1),   */
+/* set_halo! before every operator application, the per-site half-spin       */
+/* kernel reading its neighbours from the halo, unfused BLAS-1 over the core  */
+/* sites, reductions per thread then summed.  CPU baseline only: bench.py     */
+/* times it beside the GPU solve.                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int L[4], P[4];
+    long V, Vp, pstride[4];
+    cx ph[4], iph[4];
+} pgeom;
+
+static void pgeom_init(pgeom* g, const int* L, const double* phases) {
+    g->V = 1;
+    g->Vp = 1;
+    for (int d = 0; d < 4; ++d) {
+        g->L[d] = L[d];
+        g->P[d] = L[d] + 2;
+        g->pstride[d] = g->Vp;
+        g->V *= L[d];
+        g->Vp *= g->P[d];
+        g->ph[d] = C(phases ? phases[2 * d] : 1.0, phases ? phases[2 * d + 1] : 0.0);
+        g->iph[d] = cinv(g->ph[d]);
+    }
+}
+
+/* padded index of core site s (x fastest) */
+static inline long pcore(const pgeom* g, long s) {
+    long ps = 0;
+    for (int d = 0; d < 4; ++d) {
+        ps += (long)(s % g->L[d] + 1) * g->pstride[d];
+        s /= g->L[d];
+    }
+    return ps;
+}
+
+/* set_halo! / exchange_dim_local! on one rank (src/LatticeMatrices_core.jl:555-569,707-723):
+ * direction by direction over the whole padded extent of the others, the minus
+ * ghost = the last core slice times inv(phase), the plus ghost = the first
+ * times phase (skipped for phase 1); ne complex entries per site. */
+static void set_halo(const pgeom* g, double* a, int ne, int with_phases) {
+    for (int d = 0; d < 4; ++d) {
+        const long sd = g->pstride[d];
+        const long nslab = g->Vp / g->P[d];
+        const int skip = !with_phases || (g->ph[d].re == 1.0 && g->ph[d].im == 0.0);
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < nslab; ++q) {
+            /* q enumerates the padded sites with x_d = 0 */
+            const long lo = q % sd, hi = q / sd;
+            const long base = lo + hi * sd * g->P[d];
+            double* gm = a + 2 * ne * base;
+            double* gp = a + 2 * ne * (base + (long)(g->L[d] + 1) * sd);
+            const double* cm = a + 2 * ne * (base + (long)g->L[d] * sd);
+            const double* cp = a + 2 * ne * (base + sd);
+            for (int k = 0; k < ne; ++k) {
+                cx vm = C(cm[2 * k], cm[2 * k + 1]), vp = C(cp[2 * k], cp[2 * k + 1]);
+                if (!skip) {
+                    vm = cmul(vm, g->iph[d]);
+                    vp = cmul(vp, g->ph[d]);
+                }
+                gm[2 * k] = vm.re;
+                gm[2 * k + 1] = vm.im;
+                gp[2 * k] = vp.re;
+                gp[2 * k + 1] = vp.im;
+            }
+        }
+    }
+}
+
+/* link U_mu at padded site ps from the column-major A[3,3,...] arrays */
+static inline void load_link_pad(const double* const* U, int mu, long ps, cx u[9]) {
+    const double* p = U[mu] + ps * 18;
+    for (int row = 0; row < 3; ++row)
+        for (int col = 0; col < 3; ++col) u[row * 3 + col] = C(p[2 * (row + 3 * col)], p[2 * (row + 3 * col) + 1]);
+}
+
+/* mul!(C, D, psi) or mul!(C, D', psi): set_halo!(psi), then the half-spin kernel per core site */
+static void refcpu_wilson(const pgeom* g, double* out, double* in, const double* const* U, double kappa, int dagger) {
+    set_halo(g, in, 12, 1);
+    const int fpm = dagger ? 1 : -1;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g->V; ++s) {
+        const long ps = pcore(g, s);
+        cx acc[12], sp[12], u[9], h[6], r[6];
+        for (int k = 0; k < 12; ++k) acc[k] = C(0.0, 0.0);
+        for (int mu = 0; mu < 4; ++mu) {
+            load_spinor(in, ps + g->pstride[mu], sp);
+            for (int c = 0; c < 3; ++c) half_project(sp, c, fpm, mu, &h[2 * c], &h[2 * c + 1]);
+            load_link_pad(U, mu, ps, u);
+            for (int a = 0; a < 3; ++a) {
+                r[2 * a] = muladdmulti(u[3 * a], h[0], u[3 * a + 1], h[2], u[3 * a + 2], h[4]);
+                r[2 * a + 1] = muladdmulti(u[3 * a], h[1], u[3 * a + 1], h[3], u[3 * a + 2], h[5]);
+            }
+            reconstruct_add(acc, r, fpm, mu);
+            const long sm = ps - g->pstride[mu];
+            load_spinor(in, sm, sp);
+            for (int c = 0; c < 3; ++c) half_project(sp, c, -fpm, mu, &h[2 * c], &h[2 * c + 1]);
+            load_link_pad(U, mu, sm, u);
+            for (int a = 0; a < 3; ++a) {
+                cx w0 = cconj(u[a]), w1 = cconj(u[3 + a]), w2 = cconj(u[6 + a]);
+                r[2 * a] = muladdmulti(w0, h[0], w1, h[2], w2, h[4]);
+                r[2 * a + 1] = muladdmulti(w0, h[1], w1, h[3], w2, h[5]);
+            }
+            reconstruct_add(acc, r, -fpm, mu);
+        }
+        double* o = out + ps * 24;
+        const double* p = in + ps * 24;
+        for (int spin = 0; spin < 4; ++spin)
+            for (int c = 0; c < 3; ++c) {
+                cx v = cadd(C(p[2 * (spin * 3 + c)], p[2 * (spin * 3 + c) + 1]), rmul(-kappa, acc[4 * c + spin]));
+                o[2 * (spin * 3 + c)] = v.re;
+                o[2 * (spin * 3 + c) + 1] = v.im;
+            }
+    }
+}
+
+/* dot over the core sites: per-site sums, per-thread partial sums, then summed */
+static double refcpu_dot_re(const pgeom* g, const double* a, const double* b) {
+    double acc = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : acc)
+    for (long s = 0; s < g->V; ++s) acc += site_dot(a, b, pcore(g, s), 12).re;
+    return acc;
+}
+
+/* axpby!(a, X, b, Y) over the core sites */
+static void refcpu_axpby(const pgeom* g, double a, const double* X, double b, double* Y) {
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g->V; ++s) {
+        const long o = pcore(g, s) * 24;
+        for (int k = 0; k < 24; ++k) Y[o + k] = a * X[o + k] + b * Y[o + k];
+    }
+}
+
+/* cg!(x, DdagDOp(D, temp), b, r, p, Ap) (src/LinearAlgebras/cg.jl:73-137) on
+ * padded fields: spinors [Vp][4][3] complex (LatticeMatrix.A, nw = 1), links
+ * four column-major [Vp][3][3] arrays with their periodic halo filled. */
+int orc_refcpu_cg_ddagd(double* x, const double* b, double* r, double* p, double* Ap, double* tmp,
+                        const double* const* U, const int* L, const double* phases, double kappa, double rtol,
+                        int maxiter, orc_cg_result* res) {
+    pgeom g;
+    pgeom_init(&g, L, phases);
+    double bb = refcpu_dot_re(&g, b, b);
+    double bnorm = sqrt(bb), tol = rtol * bnorm;
+    refcpu_wilson(&g, tmp, x, U, kappa, 0);
+    refcpu_wilson(&g, Ap, tmp, U, kappa, 1);
+    memcpy(r, b, sizeof(double) * 24 * g.Vp);
+    refcpu_axpby(&g, -1.0, Ap, 1.0, r);
+    memcpy(p, r, sizeof(double) * 24 * g.Vp);
+    double rr = refcpu_dot_re(&g, r, r);
+    double rn = sqrt(rr);
+    if (rn <= tol) {
+        set_result(res, 1, 0, rn, bnorm, ORC_CONVERGED);
+        return 0;
+    }
+    for (int it = 1; it <= maxiter; ++it) {
+        refcpu_wilson(&g, tmp, p, U, kappa, 0);
+        refcpu_wilson(&g, Ap, tmp, U, kappa, 1);
+        const double pAp = refcpu_dot_re(&g, p, Ap);
+        if (!(pAp > 0.0)) {
+            set_result(res, 0, it - 1, rn, bnorm, isfinite(pAp) ? ORC_NONPOSITIVE_CURVATURE : ORC_BREAKDOWN);
+            return 0;
+        }
+        const double alpha = rr / pAp;
+        refcpu_axpby(&g, alpha, p, 1.0, x);
+        refcpu_axpby(&g, -alpha, Ap, 1.0, r);
+        const double rr_new = refcpu_dot_re(&g, r, r);
+        rn = sqrt(rr_new);
+        if (rn <= tol) {
+            set_result(res, 1, it, rn, bnorm, ORC_CONVERGED);
+            return 0;
+        }
+        refcpu_axpby(&g, 1.0, r, rr_new / rr, p);
+        rr = rr_new;
+    }
+    set_result(res, 0, maxiter, rn, bnorm, ORC_MAXITER);
+    return 0;
+}
+
+/* LatticeMatrix(A, dim, PEs; nw = 1) of a core spinor [V][4][3] or of four
+ * core link fields [4][V][3][3] (row-major in, column-major padded out). */
+void orc_refcpu_pad_spinor(double* dst, const double* src, const int* L, const double* phases) {
+    pgeom g;
+    pgeom_init(&g, L, phases);
+    memset(dst, 0, sizeof(double) * 24 * g.Vp);
+    for (long s = 0; s < g.V; ++s) memcpy(dst + pcore(&g, s) * 24, src + s * 24, sizeof(double) * 24);
+    set_halo(&g, dst, 12, 1);
+}
+void orc_refcpu_pad_links(double* const* dst, const double* src, const int* L) {
+    pgeom g;
+    pgeom_init(&g, L, NULL);
+    for (int mu = 0; mu < 4; ++mu) {
+        memset(dst[mu], 0, sizeof(double) * 18 * g.Vp);
+        for (long s = 0; s < g.V; ++s) {
+            const double* a = src + ((long)mu * g.V + s) * 18;
+            double* o = dst[mu] + pcore(&g, s) * 18;
+            for (int row = 0; row < 3; ++row)
+                for (int col = 0; col < 3; ++col) {
+                    o[2 * (row + 3 * col)] = a[2 * (row * 3 + col)];
+                    o[2 * (row + 3 * col) + 1] = a[2 * (row * 3 + col) + 1];
+                }
+        }
+        set_halo(&g, dst[mu], 9, 0);
+    }
+}
+void orc_refcpu_unpad_spinor(double* dst, const double* src, const int* L) {
+    pgeom g;
+    pgeom_init(&g, L, NULL);
+    for (long s = 0; s < g.V; ++s) memcpy(dst + s * 24, src + pcore(&g, s) * 24, sizeof(double) * 24);
 }
 
 int orc_num_threads(void) {

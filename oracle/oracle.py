@@ -34,7 +34,7 @@ class _OrcOp(ctypes.Structure):
                 ("L", ctypes.c_int * 4), ("phases", ctypes.c_double * 8),
                 ("kappa", ctypes.c_double), ("csw", ctypes.c_double),
                 ("scale_re", ctypes.c_double), ("scale_im", ctypes.c_double),
-                ("tmp", ctypes.c_void_p)]
+                ("tmp", ctypes.c_void_p), ("nc", ctypes.c_int)]
 
 
 class _OrcResult(ctypes.Structure):
@@ -85,6 +85,17 @@ def lib():
         L.orc_site_rng_draw.argtypes = [U64, ctypes.c_int, U64, ctypes.c_int, P, U32, P, P, P, P, P, P]
         L.orc_randomize_fill.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P,
                                          ctypes.c_int, ctypes.c_double, U64, ctypes.c_int]
+        L.orc_dot_n.argtypes = [P, P, ctypes.c_long, ctypes.c_int, ctypes.c_int, P]
+        L.orc_wilson_nc.argtypes = [P, P, P, P, P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_clover_field_strength_nc.argtypes = [P, P, P, ctypes.c_int]
+        L.orc_wilson_clover_nc.argtypes = [P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_int]
+        L.orc_wilson_pullback_nc.argtypes = [P, P, P, P, P, ctypes.c_double, ctypes.c_int]
+        L.orc_refcpu_cg_ddagd.argtypes = [P, P, P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                          ctypes.POINTER(_OrcResult)]
+        L.orc_refcpu_pad_spinor.argtypes = [P, P, P, P]
+        L.orc_refcpu_pad_links.argtypes = [P, P, P]
+        L.orc_refcpu_unpad_spinor.argtypes = [P, P, P]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -170,17 +181,19 @@ class NormalOp:
     """Operator description for the oracle CG: D^dagger D of Wilson/clover, a
     complex scale (mul!(y, s, x)) or D^dagger D of a scale."""
 
-    def __init__(self, kind, L, U=None, F=None, kappa=0.0, csw=0.0, phases=(1, 1, 1, -1), scale=0j):
+    def __init__(self, kind, L, U=None, F=None, kappa=0.0, csw=0.0, phases=(1, 1, 1, -1), scale=0j, nc=3):
         self.V = volume(L)
+        self.nc = int(nc)
         self._keep = []
         self.s = _OrcOp()
         self.s.kind = kind
+        self.s.nc = self.nc
         if U is not None:
-            U = _c(U, (4, self.V, 3, 3))
+            U = _c(U, (4, self.V, nc, nc))
             self._keep.append(U)
             self.s.U = U.ctypes.data
         if F is not None:
-            F = _c(F, (6, self.V, 3, 3))
+            F = _c(F, (6, self.V, nc, nc))
             self._keep.append(F)
             self.s.F = F.ctypes.data
         for d in range(4):
@@ -190,12 +203,12 @@ class NormalOp:
             self.s.phases[k] = ph[k]
         self.s.kappa, self.s.csw = float(kappa), float(csw)
         self.s.scale_re, self.s.scale_im = complex(scale).real, complex(scale).imag
-        tmp = np.zeros((self.V, 4, 3), dtype=np.complex128)
+        tmp = np.zeros((self.V, 4, nc), dtype=np.complex128)
         self._keep.append(tmp)
         self.s.tmp = tmp.ctypes.data
 
     def apply(self, x):
-        x = _c(x, (self.V, 4, 3))
+        x = _c(x, (self.V, 4, self.nc))
         y = np.empty_like(x)
         lib().orc_op_apply(ctypes.byref(self.s), _ptr(y), _ptr(x), self.V)
         return y
@@ -204,8 +217,8 @@ class NormalOp:
 def cg(op: NormalOp, b, x0=None, rtol=1e-10, atol=0.0, maxiter=5000, order=1):
     """cg!(x, A, b, r, p, Ap; rtol, atol, maxiter); returns (x, CGResult)."""
     V = op.V
-    b = _c(b, (V, 4, 3))
-    x = np.zeros_like(b) if x0 is None else _c(x0, (V, 4, 3)).copy()
+    b = _c(b, (V, 4, op.nc))
+    x = np.zeros_like(b) if x0 is None else _c(x0, (V, 4, op.nc)).copy()
     r = np.zeros_like(b)
     p = np.zeros_like(b)
     Ap = np.zeros_like(b)
@@ -214,6 +227,69 @@ def cg(op: NormalOp, b, x0=None, rtol=1e-10, atol=0.0, maxiter=5000, order=1):
                  int(maxiter), int(order), ctypes.byref(res))
     return x, CGResult(bool(res.converged), int(res.iterations), float(res.residual_norm),
                        float(res.relative_residual), REASONS[int(res.reason)])
+
+
+# ---------------------------------------------------------------------------
+# Generic NC (lmx_nc_oracle.c): the reference's kernels for NC != 3.  Arrays
+# as above with nc in place of 3: psi[V, 4, nc], U[4, V, nc, nc],
+# F[6, V, nc, nc].
+# ---------------------------------------------------------------------------
+def wilson_nc(U, psi, L, kappa, phases=(1, 1, 1, -1), dagger=False, donly=False):
+    """D psi for any NC by the reference's generic kernels (NC = 3 runs them too,
+    although the reference itself uses its half-spin kernels there)."""
+    V = volume(L)
+    psi = _c(psi)
+    nc = psi.shape[-1]
+    psi = psi.reshape(V, 4, nc)
+    U = _c(U, (4, V, nc, nc))
+    out = np.empty_like(psi)
+    lib().orc_wilson_nc(_ptr(out), _ptr(psi), _ptr(U), _dims(L), _ptr(_phases(phases)), float(kappa), nc,
+                        int(dagger), int(donly))
+    return out
+
+
+def clover_field_strength_nc(U, L):
+    V = volume(L)
+    U = _c(U)
+    nc = U.shape[-1]
+    U = U.reshape(4, V, nc, nc)
+    F = np.empty((6, V, nc, nc), dtype=np.complex128)
+    lib().orc_clover_field_strength_nc(_ptr(F), _ptr(U), _dims(L), nc)
+    return F
+
+
+def wilson_clover_nc(U, F, psi, L, kappa, csw, phases=(1, 1, 1, -1), dagger=False):
+    V = volume(L)
+    psi = _c(psi)
+    nc = psi.shape[-1]
+    psi = psi.reshape(V, 4, nc)
+    U = _c(U, (4, V, nc, nc))
+    F = _c(F, (6, V, nc, nc))
+    out = np.empty_like(psi)
+    lib().orc_wilson_clover_nc(_ptr(out), _ptr(psi), _ptr(U), _ptr(F), _dims(L), _ptr(_phases(phases)),
+                               float(kappa), float(csw), nc, int(dagger))
+    return out
+
+
+def wilson_pullback_nc(dres, psi, L, coefficient, phases=(1, 1, 1, -1), dU=None):
+    V = volume(L)
+    psi = _c(psi)
+    nc = psi.shape[-1]
+    psi = psi.reshape(V, 4, nc)
+    dres = _c(dres, (V, 4, nc))
+    dU = np.zeros((4, V, nc, nc), dtype=np.complex128) if dU is None else _c(dU, (4, V, nc, nc)).copy()
+    lib().orc_wilson_pullback_nc(_ptr(dU), _ptr(dres), _ptr(psi), _dims(L), _ptr(_phases(phases)),
+                                 float(coefficient), nc)
+    return dU
+
+
+def dot_n(a, b, nc, order=1):
+    a = _c(a)
+    b = _c(b)
+    V = a.size // (4 * nc)
+    out = np.zeros(2, dtype=np.float64)
+    lib().orc_dot_n(_ptr(a), _ptr(b), V, 4 * nc, int(order), _ptr(out))
+    return complex(out[0], out[1])
 
 
 def wilson_pullback(dres, psi, L, coefficient, phases=(1, 1, 1, -1), dU=None):
@@ -347,3 +423,33 @@ def randomize_fill(dtype, nc1, nc2, L, key, gaussian=False, sigma=1.0, algorithm
 
 def volume_any(L):
     return int(np.prod([int(v) for v in L]))
+
+
+# ---------------------------------------------------------------------------
+# The reference's CPU algorithm (padded storage, halo refresh per operator
+# application, unfused BLAS-1): the CPU baseline of bench.py
+# ---------------------------------------------------------------------------
+def refcpu_cg_ddagd(U, b, L, kappa, phases=(1, 1, 1, -1), rtol=1e-10, maxiter=5000):
+    """cg! of D^dagger D as the reference runs it on the CPU, from a zero start.
+    Returns (x [V, 4, 3], CGResult, seconds of the solve itself)."""
+    import time
+    V = volume(L)
+    Vp = int(np.prod([int(d) + 2 for d in L]))
+    U = _c(U, (4, V, 3, 3))
+    b = _c(b, (V, 4, 3))
+    ph = _phases(phases)
+    Up = [np.zeros((Vp, 3, 3), dtype=np.complex128) for _ in range(4)]
+    uptr = (ctypes.c_void_p * 4)(*[u.ctypes.data for u in Up])
+    lib().orc_refcpu_pad_links(uptr, _ptr(U), _dims(L))
+    bp = np.zeros((Vp, 4, 3), dtype=np.complex128)
+    lib().orc_refcpu_pad_spinor(_ptr(bp), _ptr(b), _dims(L), _ptr(ph))
+    x, r, p, Ap, tmp = (np.zeros_like(bp) for _ in range(5))
+    res = _OrcResult()
+    t0 = time.perf_counter()
+    lib().orc_refcpu_cg_ddagd(_ptr(x), _ptr(bp), _ptr(r), _ptr(p), _ptr(Ap), _ptr(tmp), uptr, _dims(L), _ptr(ph),
+                              float(kappa), float(rtol), int(maxiter), ctypes.byref(res))
+    dt = time.perf_counter() - t0
+    xc = np.zeros((V, 4, 3), dtype=np.complex128)
+    lib().orc_refcpu_unpad_spinor(_ptr(xc), _ptr(x), _dims(L))
+    return xc, CGResult(bool(res.converged), int(res.iterations), float(res.residual_norm),
+                        float(res.relative_residual), REASONS[int(res.reason)]), dt

@@ -32,7 +32,7 @@ def _stub_operator(U):
 
 
 def test_refresh_decision():
-    lat = types.SimpleNamespace(V=16, dims=(2, 2, 2, 2), device=torch.device("cpu"))
+    lat = types.SimpleNamespace(V=16, dims=(2, 2, 2, 2), device=torch.device("cpu"), nc=3, ncomp=12)
     U = lmx.GaugeField(lat)
     op = _stub_operator(U)
     _ensure_clover_cache_current(op, U)

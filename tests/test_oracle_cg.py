@@ -80,3 +80,18 @@ def test_orders_agree_on_solution():
     x1, r1 = orc.cg(op, b, rtol=1e-10, order=1)
     assert r0.iterations == r1.iterations
     assert np.max(np.abs(x0 - x1)) <= 1e-12 * np.max(np.abs(x0))
+
+
+def test_reference_cpu_path_matches_oracle_cg():
+    # bench.py's CPU baseline (padded storage, set_halo! per operator
+    # application, unfused BLAS-1, per-thread reductions) solves the same
+    # system as the oracle's CG: same iterations, the same solution to rounding
+    L = (4, 4, 4, 8)
+    U = gen.links(L, "haar", 91)
+    b = gen.gaussian_spinor(L, 92)
+    x, res, _ = orc.refcpu_cg_ddagd(U, b, L, 0.125)
+    op = orc.NormalOp(orc.OP_DDAGD_WILSON, L, U=U, kappa=0.125)
+    xo, ro = orc.cg(op, b, rtol=1e-10, order=0)
+    assert res.converged and res.iterations == ro.iterations
+    assert np.max(np.abs(x - xo)) <= 1e-11 * np.max(np.abs(xo))
+    assert np.linalg.norm(op.apply(x) - b) <= 1.01e-10 * np.linalg.norm(b)
