@@ -87,7 +87,9 @@ typedef struct {
     int32_t use_graph;        /* capture check_every iterations in a hipGraph */
     int32_t rhs_block;        /* inner dslash shape (0: autotuned): (ZS << 20) | (TL << 12) | (RHS per launch << 4)
                                  | RHS per work item (1, 2, 4); TL = t-slices per streaming workgroup, ZS = 3-D
-                                 sub-ranges of the lattice (one XCD's slab spans one), 0 automatic for both */
+                                 sub-ranges of the lattice (one XCD's slab spans one), 0 automatic for both;
+                                 bit 28 set: the RHS-loop shape (FP16 links), (1 << 28) | (RHS per launch << 4) | 1,
+                                 each work item applying its site's links to every RHS of the launch */
     int32_t reserved[4];
 } lmx_cg_params;
 

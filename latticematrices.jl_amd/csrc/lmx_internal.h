@@ -131,6 +131,16 @@ int launch_wilson_padded_f32(lmx_context* ctx, const float2* U, const float2* cl
 // the marked terms and its own alpha p -- one x_inner read and write per
 // group instead of one per iteration.
 constexpr int kXBatch = 4;
+// rhs_block bit of the RHS-loop shape: each work item loads its site's eight
+// FP16 links once and applies them to every RHS of the launch in turn.
+constexpr int kRhsLoop = 1 << 28;
+// Whether a launch can take the RHS-loop shape on this context (3-D volume a
+// multiple of 256, boundary phases 1 in x, y, z and +-1 in t).
+inline bool rhsloop_ok(const lmx_context* ctx) {
+    const Geom& g = ctx->g;
+    const long V3 = (long)g.L[0] * g.L[1] * g.L[2];
+    return V3 % 256 == 0 && !g.general && g.phase_one[0] && g.phase_one[1] && g.phase_one[2];
+}
 struct LoArgs {
     int link_prec, spinor_prec;
     const void* links;

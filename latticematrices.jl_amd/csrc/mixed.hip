@@ -716,6 +716,10 @@ static std::string arch_name(int dev) {
 // dividing the 256-site blocks of a t-slice, streaming fields only where the
 // lattice streams.  Anything else (a foreign or damaged line) is retuned.
 static bool block_launchable(const lmx_context* c, int blk) {
+    if ((blk & kRhsLoop) != 0) {  // the RHS-loop shape: RHS per launch only
+        const int rest = blk & ~kRhsLoop;
+        return rhsloop_ok(c) && (rest & ~(0xff << 4)) == 1 && ((rest >> 4) & 0xff) >= 1;
+    }
     if (blk <= 0 || (blk >> 24) != 0) return false;
     const int nr = blk & 15, per = (blk >> 4) & 0xff, tl = (blk >> 12) & 0xff, zs = (blk >> 20) & 0xf;
     if (nr != 1 && nr != 2 && nr != 4) return false;
@@ -917,6 +921,26 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
             }
         }
         }
+        }
+    }
+    // the RHS-loop shape (FP16 links): RHS per launch only
+    for (int pi = 0; pi < 5 && rc == LMX_OK && precision != LMX_F32 && rhsloop_ok(c); ++pi) {
+        const int per = per_opts[pi];
+        if (per > nrhs) break;
+        const int blk = kRhsLoop | (per << 4) | 1;
+        ld.rhs_block = lx.rhs_block = blk;
+        rc = group();
+        const int reps = 2;
+        (void)hipEventRecord(e0, c->stream);
+        for (int k = 0; k < reps && rc == LMX_OK; ++k) rc = group();
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = LMX_ERR_HIP;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double us = 1e3 * ms / reps / kXBatch / nrhs / 2;
+        if (rc == LMX_OK && us < best_us) {
+            best_us = us;
+            best = blk;
         }
     }
     (void)hipEventDestroy(e0);

@@ -519,6 +519,111 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
 }
 
 // ---------------------------------------------------------------------------
+// RHS-loop kernel (FP16 links): one work item per site of a 256-site block at
+// one t-slice.  Its eight links (U_mu(x) and U_mu(x - mu)) are loaded once, as
+// the packed FP16 words they are stored in (72 registers), and applied to
+// every right-hand side of the launch in turn, so a launch over n RHS reads
+// each link once instead of once per RHS group.  Each RHS's arithmetic is
+// k_site's (the eight half-direction hops of hop_half in the reference order,
+// then finish_site and a per-block partial at block b3 + nblk3 t), so the
+// result equals k_site's bit for bit.  Workgroups are ordered t fastest
+// after the XCD remap: the t-neighbour slices of a block run on the same XCD
+// at the same time.
+// ---------------------------------------------------------------------------
+struct LinkWords {
+    int w[9];
+};
+__device__ __forceinline__ LinkWords ld_link_words(const LinkF16& U, int mu, int s) {
+    const int4 a = ldv(U.p + (size_t)(2 * mu) * U.V, s);
+    const int4 b = ldv(U.p + (size_t)(2 * mu + 1) * U.V, s);
+    const int c = ldv((const int*)(U.p + (size_t)8 * U.V) + (size_t)mu * U.V, s);
+    return LinkWords{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c}};
+}
+
+template <int MU, int PM, bool FWD, class In>
+__device__ __forceinline__ void hop_words(cx<float> (&acc)[12], const In& in, const LinkWords& lw, int r, int s0,
+                                          const int x[4], const Geom& g) {
+    // the site index "changes" here: the neighbour's address offsets are
+    // recomputed per hop (a few integer operations) instead of 36 per-lane
+    // offsets being kept live across the RHS loop
+    int s = s0;
+    asm volatile("" : "+v"(s));
+    bool w;
+    const int t = FWD ? nbr_fwd(s, x, MU, g, w) : nbr_bwd(s, x, MU, g, w);
+    const bool ng = w && g.phase_neg[MU];
+    cx<float> u[9], psi[12], h[6], o[6];
+    // the link words "change" here, after the previous hop's register pin:
+    // their FP16 -> FP32 conversions stay with this hop instead of all eight
+    // links' conversions being scheduled at the top of the RHS loop
+    int lw9[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        lw9[k] = lw.w[k];
+        asm volatile("" : "+v"(lw9[k]));
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const float2 v = h2f(lw9[k]);
+        u[k] = mk<float>(v.x, v.y);
+    }
+    float c;
+    in.ld12s(r, t, psi, c);
+    half_project<PM, MU>(psi, h);
+    if (MU == 3) neg_select(h, ng);  // low-precision launches: only t can wrap with a sign
+    if (FWD)
+        matvec_fwd(u, h, o);
+    else
+        matvec_bwd(u, h, o);
+    reconstruct_add_s<PM, MU>(acc, o, c);
+    pin_acc(acc);
+}
+
+template <class In, class Out, class Cl, class Aux, class Ex, int FPM, int MODE, int EPI>
+__global__ void __launch_bounds__(256) k_rhsloop(Out out, In in, LinkF16 U, Cl F, Aux aux, Ex ex, Geom g,
+                                                 double kappa_d, double ccoef_d, int nrhs, EpiArgs epi) {
+    if (epi.gate != nullptr && *epi.gate == 0) return;
+    __shared__ double lds4[4];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int nblk3 = g.stride[3] >> 8;
+    const int t = lb % g.L[3];
+    const int b3 = lb / g.L[3];
+    const int s = (b3 << 8) + threadIdx.x + g.stride[3] * t;
+    int x[4];
+    site_coords(s, g, x);
+    bool w;
+    LinkWords lw[8];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+        lw[2 * mu] = ld_link_words(U, mu, s);
+        lw[2 * mu + 1] = ld_link_words(U, mu, nbr_bwd(s, x, mu, g, w));
+    }
+    const float kappa = float(kappa_d), ccoef = float(ccoef_d);
+    const int part = b3 + nblk3 * t;
+    for (int r = 0; r < nrhs; ++r) {
+        cx<float> acc[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[k] = mk<float>(0.f, 0.f);
+        hop_words<0, FPM, true>(acc, in, lw[0], r, s, x, g);
+        hop_words<0, -FPM, false>(acc, in, lw[1], r, s, x, g);
+        hop_words<1, FPM, true>(acc, in, lw[2], r, s, x, g);
+        hop_words<1, -FPM, false>(acc, in, lw[3], r, s, x, g);
+        hop_words<2, FPM, true>(acc, in, lw[4], r, s, x, g);
+        hop_words<2, -FPM, false>(acc, in, lw[5], r, s, x, g);
+        hop_words<3, FPM, true>(acc, in, lw[6], r, s, x, g);
+        hop_words<3, -FPM, false>(acc, in, lw[7], r, s, x, g);
+        int so = s;  // the own-site offsets likewise, once per RHS
+        asm volatile("" : "+v"(so));
+        cx<float> psi[12];
+        if (MODE != MODE_DONLY) in.ld12(r, so, psi);
+        const double red = finish_site<MODE, EPI>(out, psi, F, aux, ex, r, so, g, kappa, ccoef, acc, true);
+        if constexpr (EPI != EPI_NONE) {
+            const double tot = block_tree(red, lds4);
+            if (threadIdx.x == 0) epi.partials[(long)r * epi.pstride + part] = tot;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 t-streaming kernel with two lanes per site (512-thread workgroups over
 // the same 256-site blocks).  Lane j of a pair (j = thread & 1) computes
 // half-spinor component j of every projection: it loads spin j and the
@@ -959,6 +1064,72 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
             set_error("fused epilogue %d is not built for low-precision spinors", a.epi);
             return LMX_ERR_UNSUPPORTED;
         }
+    }
+    return LMX_OK;
+}
+
+// The RHS-loop shape (rhs_block bit kRhsLoop): (rhs_block >> 4) & 0xff RHS
+// per launch, all of them looped over inside each work item (FP16 links,
+// lattices whose 3-D volume is a multiple of 256 and whose phases are +-1;
+// others fall back to the per-RHS shapes).
+
+template <class In, class Out, class Cl>
+int launch_lo_loop(lmx_context* ctx, const LoArgs& a, LinkF16 U, Cl F, bool clover) {
+    const double ccoef = (-a.kappa) * a.csw;
+    int per_launch = (a.rhs_block >> 4) & 0xff;
+    if (per_launch <= 0) per_launch = a.nrhs;
+    const int V = (int)ctx->V;
+    const bool dag = (a.flags & LMX_DAGGER) != 0;
+    const int nb = (int)(ctx->V / 256);
+    hipStream_t st = ctx->stream;
+    for (int r0 = 0; r0 < a.nrhs; r0 += per_launch) {
+        const int n = std::min(per_launch, a.nrhs - r0);
+        Out o = Out::at(a.out, a.out_norm, V, r0);
+        In i = In::at(a.in, a.in_norm, V, r0);
+        EpiArgs epi{a.partials ? a.partials + (long)r0 * a.pstride : nullptr, a.pstride, nullptr, a.gate};
+        if (a.epi == EPI_XR) {
+            In p = In::at(a.aux, a.aux_norm, V, r0);
+            XrEx<In, Out> ex{In::at(a.r, a.r_norm, V, r0),
+                             Out::at(a.r, a.r_norm, V, r0),
+                             SpX4In::at(a.x, nullptr, V, r0),
+                             SpX4Out::at(a.x, nullptr, V, r0),
+                             {},
+                             a.alpha + (long)r0 * a.astride,
+                             a.alpha_q + (long)r0 * a.astride,
+                             a.astride,
+                             a.done + (long)r0 * a.dstride,
+                             a.pmask + (long)r0 * a.dstride,
+                             a.dstride,
+                             a.apply_x};
+            for (int j = 0; j < kXBatch - 1; ++j)
+                ex.pq[j] = In::at(a.pq[j] ? a.pq[j] : a.aux, a.pq[j] ? a.pq_norm[j] : a.aux_norm, V, r0);
+            if (!dag) {
+                set_error("the fused r / x update runs on D^dagger");
+                return LMX_ERR_UNSUPPORTED;
+            }
+            if (clover)
+                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, XrEx<In, Out>, 1, MODE_CLOVER, EPI_XR>), dim3(nb),
+                                   dim3(256), 0, st, o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, epi);
+            else
+                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, XrEx<In, Out>, 1, MODE_WILSON, EPI_XR>), dim3(nb),
+                                   dim3(256), 0, st, o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, epi);
+        } else if (a.epi == EPI_NORM) {
+            if (dag) {
+                set_error("|D p|^2 runs on D");
+                return LMX_ERR_UNSUPPORTED;
+            }
+            const NoEx none{};
+            if (clover)
+                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, NoEx, -1, MODE_CLOVER, EPI_NORM>), dim3(nb), dim3(256),
+                                   0, st, o, i, U, F, i, none, ctx->g, a.kappa, ccoef, n, epi);
+            else
+                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, NoEx, -1, MODE_WILSON, EPI_NORM>), dim3(nb), dim3(256),
+                                   0, st, o, i, U, F, i, none, ctx->g, a.kappa, ccoef, n, epi);
+        } else {
+            set_error("the RHS-loop shape is built for the mixed solver's inner stencils");
+            return LMX_ERR_UNSUPPORTED;
+        }
+        LMX_HIP(hipGetLastError());
     }
     return LMX_OK;
 }

@@ -19,5 +19,11 @@ LMX_LO_COMBO_1(LMX_INST, LMX_LO_NR)
 LMX_LO_COMBO_2(LMX_INST, LMX_LO_NR)
 #endif
 #undef LMX_INST
+// the RHS-loop shape (FP16 links), built with the one-RHS-per-work-item objects
+#if LMX_LO_NR == 1 && LMX_LO_COMBO == 1
+template int launch_lo_loop<SpF32In, SpF32Out, F16Clover>(lmx_context*, const LoArgs&, LinkF16, F16Clover, bool);
+#elif LMX_LO_NR == 1 && LMX_LO_COMBO == 2
+template int launch_lo_loop<SpH16In, SpH16Out, F16Clover>(lmx_context*, const LoArgs&, LinkF16, F16Clover, bool);
+#endif
 
 }  // namespace lmx

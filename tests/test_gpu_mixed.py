@@ -279,3 +279,31 @@ def test_low_precision_copies_follow_in_place_link_writes(gpu):
     assert [r.iterations for r in r1] == [r.iterations for r in r2]
     assert np.array_equal(x1.to_numpy(), x2.to_numpy())
     assert all(r.converged for r in r1)
+
+
+RHS_LOOP = 1 << 28
+
+
+@pytest.mark.parametrize("precision,per,csw", [("h16", 5, 0.0), ("h16", 2, 0.0), ("f16", 5, 0.0), ("h16", 5, 1.0)])
+def test_rhs_loop_shape(gpu, precision, per, csw):
+    # the RHS-loop shape (each work item applies its site's FP16 links to every
+    # RHS of the launch) against the FP64 oracle, on an uneven batch of 5 RHS
+    # split into launches of `per`, with graphs and with clover
+    L, kappa, rtol = (8, 8, 4, 8), 0.124, 1e-10
+    U, b, lat, Ug, D = setup(L, "weak", 330, kappa, csw=csw, nrhs=5)
+    print(f"RHS-loop {precision} per={per} csw={csw} L={L}", flush=True)
+    bg = lat.spinor_from_numpy(b)
+    x = lat.spinor(5)
+    res = lmx.cg_batched_(x, D, bg, precision=precision, rtol=rtol, use_graph=True,
+                          rhs_block=RHS_LOOP | (per << 4) | 1)
+    ref = lat.spinor(5)
+    rref = lmx.cg_batched_(ref, D, bg, precision=precision, rtol=rtol, rhs_block=(5 << 4) | 1)
+    checks = oracle_check(U, L, kappa, b, x.to_numpy(), rtol, csw=csw)
+    for r, q, (tr, err) in zip(res, rref, checks):
+        print(f"  iters={r.iterations} (per-RHS shape {q.iterations}) true_rel={tr:.3e} err_vs_fp64={err:.3e}",
+              flush=True)
+        assert r.converged and tr <= rtol * 1.05 and err <= 1e-7
+        assert abs(r.iterations - q.iterations) <= 3
+    # the inner kernels alone at this shape: timed without error
+    t = lmx.inner_kernel_times(D, precision, 5, RHS_LOOP | (per << 4) | 1, groups=1)
+    assert t["D"] > 0 and t["Ddag"] > 0
