@@ -191,9 +191,12 @@ struct NoEx {};
 // update is batched over groups of kXBatch iterations: with apply_x == 0 the
 // kernel leaves x alone (alpha p is deferred, p kept in its ring slot); with
 // apply_x == 1 it adds the deferred alpha_q[j] pq[j] of the marked slots,
-// oldest first, and then alpha p.
-template <class In, class Out>
+// oldest first, and then alpha p.  APPLY fixes apply_x at compile time (0:
+// defer, 1: apply), so the deferring kernel (three launches in four) is
+// compiled without the registers of the x update.
+template <class In, class Out, int APPLY>
 struct XrEx {
+    static constexpr int kApply = APPLY;
     In rin;        // r (read) ...
     Out rout;      // ... and written in place
     SpX4In xin;    // inner solution x (FP32, component pairs)
@@ -244,7 +247,7 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
         red = double(site);
         const bool live = store && ex.done[(long)r * ex.dstride] == 0;
         if (live) ex.rout.st12(r, s, rv);
-        if (live && ex.apply_x) {
+        if (Ex::kApply == 1 && live) {
             cx<R> pv[12], xv[12];
             aux.ld12(r, s, pv);
             ex.xin.ld12(r, s, xv);
@@ -1013,6 +1016,26 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
     return LMX_OK;
 }
 
+// The EPI_XR fields of a launch starting at RHS r0.
+template <class In, class Out, int APPLY>
+XrEx<In, Out, APPLY> make_xr(const LoArgs& a, int V, int r0) {
+    XrEx<In, Out, APPLY> ex{In::at(a.r, a.r_norm, V, r0),
+                            Out::at(a.r, a.r_norm, V, r0),
+                            SpX4In::at(a.x, nullptr, V, r0),
+                            SpX4Out::at(a.x, nullptr, V, r0),
+                            {},
+                            a.alpha + (long)r0 * a.astride,
+                            a.alpha_q + (long)r0 * a.astride,
+                            a.astride,
+                            a.done + (long)r0 * a.dstride,
+                            a.pmask + (long)r0 * a.dstride,
+                            a.dstride,
+                            a.apply_x};
+    for (int j = 0; j < kXBatch - 1; ++j)
+        ex.pq[j] = In::at(a.pq[j] ? a.pq[j] : a.aux, a.pq[j] ? a.pq_norm[j] : a.aux_norm, V, r0);
+    return ex;
+}
+
 // One low-precision (In, Out, Lk, Cl, NR) combination: (rhs_block >> 4) & 0xff
 // right-hand sides per launch, NR per work item, (rhs_block >> 12) & 0xff
 // t-slices per streaming workgroup, (rhs_block >> 20) & 0xf z-ranges (the
@@ -1035,22 +1058,15 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
         const NoEx none{};
         if (a.epi == EPI_XR) {
             In p = In::at(a.aux, a.aux_norm, V, r0);
-            XrEx<In, Out> ex{In::at(a.r, a.r_norm, V, r0),
-                             Out::at(a.r, a.r_norm, V, r0),
-                             SpX4In::at(a.x, nullptr, V, r0),
-                             SpX4Out::at(a.x, nullptr, V, r0),
-                             {},
-                             a.alpha + (long)r0 * a.astride,
-                             a.alpha_q + (long)r0 * a.astride,
-                             a.astride,
-                             a.done + (long)r0 * a.dstride,
-                             a.pmask + (long)r0 * a.dstride,
-                             a.dstride,
-                             a.apply_x};
-            for (int j = 0; j < kXBatch - 1; ++j)
-                ex.pq[j] = In::at(a.pq[j] ? a.pq[j] : a.aux, a.pq[j] ? a.pq_norm[j] : a.aux_norm, V, r0);
-            LMX_TRY((dispatch<In, Lk, Out, Cl, In, XrEx<In, Out>, NR, EPI_XR, false, false>(
-                ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
+            if (a.apply_x) {
+                const XrEx<In, Out, 1> ex = make_xr<In, Out, 1>(a, V, r0);
+                LMX_TRY((dispatch<In, Lk, Out, Cl, In, XrEx<In, Out, 1>, NR, EPI_XR, false, false>(
+                    ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
+            } else {
+                const XrEx<In, Out, 0> ex = make_xr<In, Out, 0>(a, V, r0);
+                LMX_TRY((dispatch<In, Lk, Out, Cl, In, XrEx<In, Out, 0>, NR, EPI_XR, false, false>(
+                    ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
+            }
         } else if (a.epi == EPI_NORM) {
             LMX_TRY((dispatch<In, Lk, Out, Cl, In, NoEx, NR, EPI_NORM, false, false>(
                 ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
@@ -1089,30 +1105,25 @@ int launch_lo_loop(lmx_context* ctx, const LoArgs& a, LinkF16 U, Cl F, bool clov
         EpiArgs epi{a.partials ? a.partials + (long)r0 * a.pstride : nullptr, a.pstride, nullptr, a.gate};
         if (a.epi == EPI_XR) {
             In p = In::at(a.aux, a.aux_norm, V, r0);
-            XrEx<In, Out> ex{In::at(a.r, a.r_norm, V, r0),
-                             Out::at(a.r, a.r_norm, V, r0),
-                             SpX4In::at(a.x, nullptr, V, r0),
-                             SpX4Out::at(a.x, nullptr, V, r0),
-                             {},
-                             a.alpha + (long)r0 * a.astride,
-                             a.alpha_q + (long)r0 * a.astride,
-                             a.astride,
-                             a.done + (long)r0 * a.dstride,
-                             a.pmask + (long)r0 * a.dstride,
-                             a.dstride,
-                             a.apply_x};
-            for (int j = 0; j < kXBatch - 1; ++j)
-                ex.pq[j] = In::at(a.pq[j] ? a.pq[j] : a.aux, a.pq[j] ? a.pq_norm[j] : a.aux_norm, V, r0);
             if (!dag) {
                 set_error("the fused r / x update runs on D^dagger");
                 return LMX_ERR_UNSUPPORTED;
             }
-            if (clover)
-                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, XrEx<In, Out>, 1, MODE_CLOVER, EPI_XR>), dim3(nb),
-                                   dim3(256), 0, st, o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, epi);
+#define LMX_LOOP_XR(AP)                                                                                         \
+    do {                                                                                                        \
+        const XrEx<In, Out, AP> ex = make_xr<In, Out, AP>(a, V, r0);                                            \
+        if (clover)                                                                                             \
+            hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, XrEx<In, Out, AP>, 1, MODE_CLOVER, EPI_XR>), dim3(nb), \
+                               dim3(256), 0, st, o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, epi);            \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, XrEx<In, Out, AP>, 1, MODE_WILSON, EPI_XR>), dim3(nb), \
+                               dim3(256), 0, st, o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, epi);            \
+    } while (0)
+            if (a.apply_x)
+                LMX_LOOP_XR(1);
             else
-                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, XrEx<In, Out>, 1, MODE_WILSON, EPI_XR>), dim3(nb),
-                                   dim3(256), 0, st, o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, epi);
+                LMX_LOOP_XR(0);
+#undef LMX_LOOP_XR
         } else if (a.epi == EPI_NORM) {
             if (dag) {
                 set_error("|D p|^2 runs on D");
