@@ -16,8 +16,8 @@ region holds device work only (plus the solver's host convergence polls).
 
 Also measured, each with HIP events on the library stream: the mixed solve's
 two inner stencil kernels at the autotuned shape (`roofline`: D^dagger with
-the fused r / x update, the dominant kernel of the timed solve; and D with
-|Dp|^2), the FP64 D_W launch (960 B/site algorithmic: links 576 + psi 192 +
+the fused r / x update, the dominant kernel of the timed solve; and D of r
+with the direction update and |Dp|^2), the FP64 D_W launch (960 B/site algorithmic: links 576 + psi 192 +
 out 192), and a plain 1 GiB copy kernel used to calibrate the rocprof byte
 counters.  The CPU baseline is one full solve of the same FP64 system by the
 reference's own CPU algorithm (oracle/: padded storage, halo refresh per
@@ -99,25 +99,26 @@ def measured_traffic(name, L, key="traffic_bytes_per_launch", nrhs=None):
 
 def inner_kernel_bytes(precision, n):
     """Algorithmic bytes per site of one launch of each inner stencil over n
-    RHS (links once per launch): D with |Dp|^2 reads p and writes t; D^dagger
-    with the fused update reads t, reads and writes r, and once in four
-    (averaged here) reads the four p of the group and reads and writes the
-    FP32 x_inner (192 B)."""
+    RHS (links once per launch): D with the direction update stencils r, reads
+    the previous t and p, writes t = D p and p; D^dagger with the fused update
+    reads t, reads and writes r, and once in four (averaged here) reads the
+    four p of the group and reads and writes the FP32 x_inner (192 B)."""
     links = 288.0 if precision == "f32" else 144.0
     spinor = 52.0 if precision == "h16" else 96.0
-    return links + n * 2 * spinor, links + n * (3 * spinor + (4 * spinor + 192.0) / 4)
+    return links + n * 5 * spinor, links + n * (3 * spinor + (4 * spinor + 192.0) / 4)
 
 
 def lo_dslash_bytes(precision, n):
     """Algorithmic bytes per site per RHS of one operator application,
     averaged as the autotuner times them: a group of four inner iterations,
-    each D with |D p|^2 (reads p, writes t) and D^dagger with the r update
-    (reads t and r, writes r), the last D^dagger also adding the four alpha p
-    (reads four p) to x_inner (FP32, read and written); links once per launch
-    of n RHS.  Per application: links / n + 3 spinors + (4 spinors + 192) / 8."""
+    each D with the direction update (stencils r, reads t and p_prev, writes t
+    and p) and D^dagger with the r update (reads t and r, writes r), the last
+    D^dagger also adding the four alpha p (reads four p) to x_inner (FP32,
+    read and written); links once per launch of n RHS.  Per application:
+    links / n + 4 spinors + (4 spinors + 192) / 8."""
     links = 288.0 if precision == "f32" else 144.0
     spinor = 52.0 if precision == "h16" else 96.0
-    group = 8 * links / n + 20 * spinor + 4 * spinor + 2 * 96.0
+    group = 8 * links / n + 32 * spinor + 4 * spinor + 2 * 96.0
     return group / 8
 
 
@@ -262,8 +263,9 @@ def main():
     bytes_dm = lo_dslash_bytes(args.precision, n)
     ach_dm = bytes_dm * V / (tune["ms_per_apply"] * 1e-3) / 1e9
     blk = tune["rhs_block"]
-    log(f"bench: autotuned {args.precision} fused D / D^dagger pair x{n}: TL {(blk >> 12) & 0xff}, z-ranges "
-        f"{blk >> 20} (0: automatic), {(blk >> 4) & 0xff} RHS per launch, {blk & 15} per work item: "
+    kind = "LDS-staged" if blk & (1 << 29) else ("RHS-loop" if blk & (1 << 28) else "streaming")
+    log(f"bench: autotuned {args.precision} fused D / D^dagger pair x{n}: {kind}, TL {(blk >> 12) & 0xff}, z-ranges "
+        f"{(blk >> 20) & 0xf} (0: automatic), {(blk >> 4) & 0xff} RHS per launch, {blk & 15} per work item: "
         f"{tune['ms_per_apply'] * 1e3:.1f} us per application per RHS, {ach_dm:.0f} GB/s algorithmic")
     kt = lmx.inner_kernel_times(D, args.precision, n, tune["rhs_block"], groups=4)
     bD, bX = inner_kernel_bytes(args.precision, n)
@@ -344,8 +346,8 @@ def main():
                "f32": "SpF32In, LinkF32, SpF32Out"}[args.precision]
     roof_X = roofline(f"k_stream<{lo_name}, ..., XrEx, +1, 0, NR, 3> (D^dagger with the r / x update, {n} RHS, "
                       f"the dominant kernel of the timed solve)", bX * V, kt["Ddag"], tr_X, tr_src)
-    roof_D = roofline(f"k_stream<{lo_name}, ..., NoEx, -1, 0, NR, 2> (D with |Dp|^2, {n} RHS)", bD * V, kt["D"],
-                      tr_D, tr_src)
+    roof_D = roofline(f"k_stream<{lo_name}, ..., TpEx, -1, 0, NR, 4> (D of r with the direction update and "
+                      f"|Dp|^2, {n} RHS)", bD * V, kt["D"], tr_D, tr_src)
     roof_64 = roofline("k_stream<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, NoEx, -1, 0, 1, 0, false> "
                        "(FP64 D_W, 32^4; the reliable updates and the FP64 CG)", bytes_d64, t_d64 * 1e3, traffic,
                        traffic_src)
@@ -378,7 +380,8 @@ def main():
         "mixed_rhs0_error_vs_fp64": err0,
         "dslash_gbs": ach_d64,
         "copy_1gib_us": t_copy * 1e3,
-        "mixed_dslash": {"kernels": "D with |Dp|^2 + D^dagger with the fused x/r update, per application",
+        "mixed_dslash": {"kernels": "D with the direction update and |Dp|^2 + D^dagger with the fused x/r "
+                                    "update, per application",
                          "rhs_block": tune["rhs_block"], "us_per_rhs": tune["ms_per_apply"] * 1e3,
                          "algorithmic_bytes_per_site_per_rhs": bytes_dm, "achieved_gbs": ach_dm,
                          "frac": ach_dm / HBM_PEAK_GBS},

@@ -89,7 +89,11 @@ typedef struct {
                                  | RHS per work item (1, 2, 4); TL = t-slices per streaming workgroup, ZS = 3-D
                                  sub-ranges of the lattice (one XCD's slab spans one), 0 automatic for both;
                                  bit 28 set: the RHS-loop shape (FP16 links), (1 << 28) | (RHS per launch << 4) | 1,
-                                 each work item applying its site's links to every RHS of the launch */
+                                 each work item applying its site's links to every RHS of the launch;
+                                 bit 29 set: the LDS-staged shape, (1 << 29) | (ZS << 20) | (TL << 12) | (RHS per
+                                 launch << 4) | 1, each workgroup holding its block's spinors at slices t and t+1
+                                 in LDS (lattices whose x extent divides 256 and whose 256-site blocks are whole
+                                 x-rows of one z-plane; elsewhere the streaming shape runs) */
     int32_t reserved[4];
 } lmx_cg_params;
 

@@ -212,12 +212,38 @@ struct SpF64Out {
         for (int k = 0; k < 12; ++k) st_out((double2*)((char*)(q + (size_t)k * V) + (unsigned)s * 16u), z[k].re, z[k].im);
     }
 };
+// A 256-site slice of one right-hand side staged in LDS by the LDS-staged
+// streaming stencil: the site's stored words, component-major, so that
+// consecutive lanes move consecutive words.
+struct F32Slot {
+    float2 w[12][256];
+};
+struct H16Slot {
+    int4 w[3][256];
+    float c[256];
+};
+
 struct SpF32In {
     using R = float;
+    using Slot = F32Slot;
     const float2* __restrict__ p;
     int V;
     static SpF32In at(const void* q, const float*, int V, int r0) {
         return SpF32In{(const float2*)q + (size_t)r0 * 12 * V, V};
+    }
+    // site s of RHS r into lane `lane` of an LDS slot
+    __device__ __forceinline__ void stage(int r, int s, Slot& sl, int lane) const {
+        const float2* q = p + (size_t)r * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) sl.w[k][lane] = ldv(q + (size_t)k * V, s);
+    }
+    __device__ static __forceinline__ void lds_ld12s(const Slot& sl, int lane, cx<float>* z, float& c) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const float2 v = sl.w[k][lane];
+            z[k] = mk<float>(v.x, v.y);
+        }
+        c = 1.0f;
     }
     __device__ __forceinline__ void ld12s(int r, int s, cx<float>* z, float& c) const {
         const float2* q = p + (size_t)r * 12 * V;
@@ -295,11 +321,31 @@ __device__ __forceinline__ int h16_pack_scaled(float re, float im, float inv) {
 
 struct SpH16In {
     using R = float;
+    using Slot = H16Slot;
     const int4* __restrict__ p;
     const float* __restrict__ scale;
     int V;
     static SpH16In at(const void* q, const float* n, int V, int r0) {
         return SpH16In{(const int4*)q + (size_t)r0 * 3 * V, n + (size_t)r0 * V, V};
+    }
+    // site s of RHS r into lane `lane` of an LDS slot
+    __device__ __forceinline__ void stage(int r, int s, Slot& sl, int lane) const {
+        const int4* q = p + (size_t)r * 3 * V;
+#pragma unroll
+        for (int g = 0; g < 3; ++g) sl.w[g][lane] = ldv(q + (size_t)g * V, s);
+        sl.c[lane] = ldv(scale + (size_t)r * V, s);
+    }
+    // integer components (as floats) and the site scale, as ld12s
+    __device__ static __forceinline__ void lds_ld12s(const Slot& sl, int lane, cx<float>* z, float& c) {
+        c = sl.c[lane];
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            const int4 v = sl.w[g][lane];
+            z[4 * g + 0] = mk<float>(h16_lo(v.x), h16_hi(v.x));
+            z[4 * g + 1] = mk<float>(h16_lo(v.y), h16_hi(v.y));
+            z[4 * g + 2] = mk<float>(h16_lo(v.z), h16_hi(v.z));
+            z[4 * g + 3] = mk<float>(h16_lo(v.w), h16_hi(v.w));
+        }
     }
     // integer components (as floats) and the site scale: value = z * c
     __device__ __forceinline__ void ld12s(int r, int s, cx<float>* z, float& c) const {

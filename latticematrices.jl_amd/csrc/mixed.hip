@@ -5,8 +5,9 @@
 // batched kernels; every scalar lives on the device):
 //   inner CG in FP32 arithmetic on FP32 or 16-bit stored vectors with FP32
 //   or FP16 links; p.Ap = |D p|^2 and |r|^2 accumulated in FP64 in the
-//   stencils' epilogues (the x/r update rides on the D^dagger launch),
-//   alpha/beta per RHS on the device;
+//   stencils' epilogues (the direction update p = r + beta p rides on the D
+//   launch, which stencils r and forms D p = D r + beta D p_prev; the x/r
+//   update rides on the D^dagger launch), alpha/beta per RHS on the device;
 //   a reliable update replaces the inner residual by the FP64 true residual
 //   r = b - A x (A = D^dagger D in FP64) and folds the inner solution into the
 //   FP64 solution x, when |r| drops below delta * (largest |r| since the
@@ -54,7 +55,7 @@ struct InnerVec {
 
 struct MixPtrs {
     float4* xlo;  // inner solution: FP32 component pairs, float4 [nrhs][6][V]
-    InnerVec r, p[kXBatch], t;  // p ring: iteration k reads p[k % kXBatch], writes p[(k + 1) % kXBatch]
+    InnerVec r, p[kXBatch], t;  // p ring: iteration k writes p[k % kXBatch] from p[(k - 1) % kXBatch]
     double2* r64;  // A x of a reliable update (FP64); the FP64 residual is formed on the fly from it
     double2* t64;  // D x, the intermediate of that A x
     double* partials;
@@ -225,11 +226,13 @@ __global__ void __launch_bounds__(256) k_mx_fin(const double* __restrict__ parti
 }
 
 // r = b - A x with A x in ax64 (FP64; r itself is not kept in FP64: the next
-// reliable update recomputes it); r_lo = p = r rounded to the inner format;
-// x_lo = 0; |r|^2 and |b|^2 per RHS.
+// reliable update recomputes it); r_lo = r rounded to the inner format; the
+// first iteration's D launch (beta = 0) forms t = D r_lo + 0 t and
+// p = r_lo + 0 p_prev from t = 0 and p_prev = r_lo written here; x_lo = 0;
+// |r|^2 and |b|^2 per RHS.
 template <class SO>
 __global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, const double2* __restrict__ ax64, SO rlo,
-                                                 SO p, float4* __restrict__ xlo, int V, double* partials) {
+                                                 SO t, SO pprev, float4* __restrict__ xlo, int V, double* partials) {
     const int i = blockIdx.y;
     const int s = blockIdx.x * 256 + threadIdx.x;
     double rr = 0.0, bb = 0.0;
@@ -246,7 +249,10 @@ __global__ void __launch_bounds__(256) k_mx_init(const double2* __restrict__ b, 
             bb += bv.x * bv.x + bv.y * bv.y;
         }
         rlo.st12(i, s, f);
-        p.st12(i, s, f);
+        pprev.st12(i, s, f);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) f[k] = mk<float>(0.f, 0.f);
+        t.st12(i, s, f);
 #pragma unroll
         for (int m = 0; m < 6; ++m) xlo[(size_t)i * 6 * V + (size_t)m * V + s] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
@@ -364,22 +370,6 @@ __global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__
     }
 }
 
-// p = r_lo + beta p
-template <class SI, class SO>
-__global__ void __launch_bounds__(256) k_mx_update_p(SI r, SI pin, SO pout, int V, const MixRhs* rhs) {
-    const int i = blockIdx.y;
-    if (rhs[i].done) return;
-    const float be = (float)rhs[i].beta;
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= V) return;
-    cx<float> rv[12], pv[12];
-    r.ld12(i, s, rv);
-    pin.ld12(i, s, pv);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) pv[k] = mk<float>(fmaf(be, pv[k].re, rv[k].re), fmaf(be, pv[k].im, rv[k].im));
-    pout.st12(i, s, pv);
-}
-
 // ---------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------
@@ -424,7 +414,10 @@ static LoArgs lo_args(lmx_wilson* op, int link_prec, int spinor_prec) {
 }
 
 // One inner iteration's operator work, in two launches with fused epilogues:
-//   t = D p, with the block partials of |t|^2 = p.Ap (FP64 sums), then
+//   t = D r + beta t = D p (beta from the previous iteration, or from the
+//   reliable update that replaced r; 0 on the first iteration), writing
+//   p = r + beta p_prev at each site, with the block partials of
+//   |t|^2 = p.Ap (FP64 sums), then
 //   (after the finalize has set alpha) Ap = D^dagger t, kept in registers:
 //   r -= alpha Ap with the block partials of |r|^2, and the x_inner update
 //   batched over groups of kXBatch iterations: phase j < kXBatch - 1 defers
@@ -434,17 +427,26 @@ static LoArgs lo_args(lmx_wilson* op, int link_prec, int spinor_prec) {
 //   update inside a group folds the marked terms into x in k_mx_accum.
 static_assert(sizeof(MixRhs) % sizeof(double) == 0, "MixRhs is addressed in doubles and ints");
 
-static int apply_d_norm(MixRun& R, int e) {
+static int apply_d_tp(MixRun& R, int e) {
     LoArgs a = lo_args(R.op, R.link_prec, R.spinor_prec);
     a.nrhs = R.nrhs;
     a.rhs_block = R.rhs_block;
     a.gate = &R.w.batch->n_active;
     a.out = R.w.t.p;
     a.out_norm = R.w.t.n;
-    a.in = R.w.p[e].p;
-    a.in_norm = R.w.p[e].n;
+    a.in = R.w.r.p;
+    a.in_norm = R.w.r.n;
+    const InnerVec& pp = R.w.p[(e + kXBatch - 1) % kXBatch];
+    a.pprev = pp.p;
+    a.pprev_norm = pp.n;
+    a.pnext = R.w.p[e].p;
+    a.pnext_norm = R.w.p[e].n;
+    a.beta = (const double*)((const char*)R.w.rhs + offsetof(MixRhs, beta));
+    a.astride = (int)(sizeof(MixRhs) / sizeof(double));
+    a.done = (const int*)((const char*)R.w.rhs + offsetof(MixRhs, done));
+    a.dstride = (int)(sizeof(MixRhs) / sizeof(int));
     a.flags = 0;
-    a.epi = 2;
+    a.epi = 4;
     a.partials = R.w.partials;
     a.pstride = 2L * nblocks(R.ctx->V);
     return launch_wilson_lo(R.ctx, a);
@@ -492,18 +494,11 @@ static void enqueue_true_res(MixRun& R, dim3 grid, int force) {
     hipLaunchKernelGGL((k_mx_true_res<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V), V,
                        R.w.partials, R.w.rhs, R.w.batch, force);
 }
-// p[(e + 1) % kXBatch] = r + beta p[e]
-template <class SI, class SO>
-static void enqueue_update_p(MixRun& R, dim3 grid, int e) {
-    const int V = (int)R.ctx->V;
-    hipLaunchKernelGGL((k_mx_update_p<SI, SO>), grid, dim3(256), 0, R.ctx->stream, in_of<SI>(R.w.r, V),
-                       in_of<SI>(R.w.p[e], V), out_of<SO>(R.w.p[(e + 1) % kXBatch], V), V, R.w.rhs);
-}
 template <class SO>
 static void enqueue_init(MixRun& R, dim3 grid) {
     const int V = (int)R.ctx->V;
     hipLaunchKernelGGL((k_mx_init<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V),
-                       out_of<SO>(R.w.p[0], V), R.w.xlo, V, R.w.partials);
+                       out_of<SO>(R.w.t, V), out_of<SO>(R.w.p[kXBatch - 1], V), R.w.xlo, V, R.w.partials);
 }
 // reliable-update part 1 (gated, or forced at the end): x += x_inner (+ the marked alpha p)
 template <class SI>
@@ -531,7 +526,7 @@ static int enqueue_iteration(MixRun& R, int e) {
     const int nb = nblocks(c->V);
     MixRhs* rhs = R.w.rhs;
     MixBatch* bt = R.w.batch;
-    LMX_TRY(apply_d_norm(R, e));
+    LMX_TRY(apply_d_tp(R, e));
     hipLaunchKernelGGL(k_mx_fin<MX_PAP>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
     LMX_TRY(apply_ddag_xr(R, e));
     hipLaunchKernelGGL(k_mx_fin<MX_RR>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, e);
@@ -542,10 +537,6 @@ static int enqueue_iteration(MixRun& R, int e) {
     else
         enqueue_true_res<SpF32Out>(R, grid, 0);
     hipLaunchKernelGGL(k_mx_fin<MX_TRUE>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
-    if (h16)
-        enqueue_update_p<SpH16In, SpH16Out>(R, grid, e);
-    else
-        enqueue_update_p<SpF32In, SpF32Out>(R, grid, e);
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }
@@ -694,9 +685,9 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
 
 // ---------------------------------------------------------------------------
 // Autotuner: time one group of kXBatch inner iterations' operator pairs as
-// the solver runs them (D with |D p|^2, D^dagger with the fused r update,
-// the x update deferred on all but the last, which adds the deferred terms
-// from their own buffers) for every RHS
+// the solver runs them (D of r with the direction update and |D p|^2,
+// D^dagger with the fused r update, the x update deferred on all but the
+// last, which adds the deferred terms from their own buffers) for every RHS
 // blocking (RHS per launch in {1,2,4,6,12}, RHS per work item in {1,2,4})
 // and streaming depth (t-slices per workgroup: automatic, 4, 8) and keep
 // the fastest per (volume, precision, nrhs, clover term or not, arch);
@@ -719,6 +710,10 @@ static bool block_launchable(const lmx_context* c, int blk) {
     if ((blk & kRhsLoop) != 0) {  // the RHS-loop shape: RHS per launch only
         const int rest = blk & ~kRhsLoop;
         return rhsloop_ok(c) && (rest & ~(0xff << 4)) == 1 && ((rest >> 4) & 0xff) >= 1;
+    }
+    if ((blk & kLdsStage) != 0) {  // the LDS-staged shape: one RHS per work item
+        if (!lds_ok(c) || (blk & 15) != 1) return false;
+        blk &= ~kLdsStage;
     }
     if (blk <= 0 || (blk >> 24) != 0) return false;
     const int nr = blk & 15, per = (blk >> 4) & 0xff, tl = (blk >> 12) & 0xff, zs = (blk >> 20) & 0xf;
@@ -763,9 +758,10 @@ static bool cache_lookup(const char* path, const lmx_context* c, int prec, int n
 
 namespace lmx {
 // The buffers and launch arguments of one group of the mixed solver's inner
-// iterations (D with |D p|^2, then D^dagger with the r update and the x update
-// deferred or applied), for timing launch shapes (lmx_autotune) and single
-// kernels (lmx_inner_kernel_times).  Freed by the destructor.
+// iterations (D of r with the direction update and |D p|^2, then D^dagger
+// with the r update and the x update deferred or applied), for timing launch
+// shapes (lmx_autotune) and single kernels (lmx_inner_kernel_times).  Freed by
+// the destructor.
 struct InnerGroup {
     char* buf = nullptr;
     LoArgs ld, lx;
@@ -780,13 +776,16 @@ static int inner_group_setup(lmx_wilson* op, int precision, int nrhs, InnerGroup
     const int sp = spinor_prec_of(precision);
     // buffers of one group of inner iterations: p, t = D p, r, x_inner,
     // partials, alpha (one per RHS and deferred slot), done (zeros), the
-    // deferred-slot masks (all slots marked) and the deferred p slots
+    // deferred-slot masks (all slots marked), the deferred p slots (the first
+    // also serves as p_prev) and beta (nonzero, so that the direction update
+    // reads t and p_prev as in every iteration after the first)
     const size_t fb = align256(lmx_spinor_bytes(c, sp, nrhs));
     const size_t xb = align256(12 * (size_t)c->V * nrhs * sizeof(float2));
     const size_t nbk = (size_t)nblocks(c->V);
     const size_t pb = align256(2 * nbk * nrhs * sizeof(double));
     const size_t ab = align256((nrhs + kXBatch) * sizeof(double)), db = align256(nrhs * sizeof(int));
-    const size_t total = 3 * fb + xb + pb + ab + 2 * db + (kXBatch - 1) * fb;
+    const size_t bb = align256(nrhs * sizeof(double));
+    const size_t total = 3 * fb + xb + pb + ab + 2 * db + (kXBatch - 1) * fb + bb;
     if (hipMalloc(&G.buf, total) != hipSuccess) {
         G.buf = nullptr;
         set_error("inner-iteration buffer allocation of %zu bytes failed", total);
@@ -803,19 +802,29 @@ static int inner_group_setup(lmx_wilson* op, int precision, int nrhs, InnerGroup
     const int* done = (const int*)((const char*)alpha + ab);
     int* masks = (int*)((char*)done + db);
     char* qbuf = (char*)masks + db;  // kXBatch - 1 deferred p slots
+    const double* beta = (const double*)(qbuf + (kXBatch - 1) * fb);
     static_assert(kXBatch >= 2 && kXBatch <= 8, "deferred-slot masks are filled bytewise");
     LMX_HIP(hipMemsetAsync(masks, (1 << (kXBatch - 1)) - 1, db, c->stream));  // slot bits 0..kXBatch-2 set
+    LMX_HIP(hipMemsetAsync((void*)beta, 0x3f, bb, c->stream));  // bytes 0x3f: beta = 4.8e-4 per RHS
     auto norm_of = [&](char* q) -> float* {  // per-site scales follow the 16-bit data
         return sp == LMX_H16 ? (float*)(q + (size_t)nrhs * c->V * 3 * sizeof(int4)) : nullptr;
     };
-    LoArgs& ld = G.ld;  // t = D p, |t|^2
+    LoArgs& ld = G.ld;  // t = D r + beta t, p = r + beta p_prev, |t|^2
     ld = lo_args(op, link_prec_of(precision), sp);
     ld.nrhs = nrhs;
-    ld.in = pbuf;
-    ld.in_norm = norm_of(pbuf);
+    ld.in = rbuf;
+    ld.in_norm = norm_of(rbuf);
     ld.out = tbuf;
     ld.out_norm = norm_of(tbuf);
-    ld.epi = 2;
+    ld.pprev = qbuf;
+    ld.pprev_norm = norm_of(qbuf);
+    ld.pnext = pbuf;
+    ld.pnext_norm = norm_of(pbuf);
+    ld.beta = beta;
+    ld.astride = 1;
+    ld.done = done;
+    ld.dstride = 1;
+    ld.epi = 4;
     ld.partials = parts;
     ld.pstride = 2L * (long)nbk;
     LoArgs& lx = G.lx;  // D^dagger t with the x / r update
@@ -923,6 +932,34 @@ extern "C" int lmx_autotune(lmx_wilson* op, int precision, int nrhs, const char*
         }
         }
     }
+    // the LDS-staged shape: RHS per launch, t-slices and z-ranges as above
+    for (int pi = 0; pi < 5 && rc == LMX_OK && lds_ok(c); ++pi) {
+        const int per = per_opts[pi];
+        if (per > nrhs) break;
+        for (int ti = 0; ti < 3 && rc == LMX_OK; ++ti) {
+            const int tl = tl_opts[ti];
+            if (tl > 0 && c->g.L[3] % tl != 0) continue;
+            for (int zi = 0; zi < 3 && rc == LMX_OK; ++zi) {
+                const int zs = zs_opts[zi];
+                if (zs > 0 && (V3 / 256) % zs != 0) continue;
+                const int blk = kLdsStage | (zs << 20) | (tl << 12) | (per << 4) | 1;
+                ld.rhs_block = lx.rhs_block = blk;
+                rc = group();
+                const int reps = 2;
+                (void)hipEventRecord(e0, c->stream);
+                for (int k = 0; k < reps && rc == LMX_OK; ++k) rc = group();
+                (void)hipEventRecord(e1, c->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) rc = LMX_ERR_HIP;
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                const double us = 1e3 * ms / reps / kXBatch / nrhs / 2;
+                if (rc == LMX_OK && us < best_us) {
+                    best_us = us;
+                    best = blk;
+                }
+            }
+        }
+    }
     // the RHS-loop shape (FP16 links): RHS per launch only
     for (int pi = 0; pi < 5 && rc == LMX_OK && precision != LMX_F32 && rhsloop_ok(c); ++pi) {
         const int per = per_opts[pi];
@@ -1002,7 +1039,7 @@ extern "C" int lmx_inner_kernel_times(lmx_wilson* op, int precision, int nrhs, i
     for (auto& e : ev) (void)hipEventDestroy(e);
     LMX_TRY(rc);
     const double per = (double)groups * kXBatch;
-    us[0] = 1e3 * tot[0] / per;  // D with |D p|^2
+    us[0] = 1e3 * tot[0] / per;  // D of r with the direction update and |D p|^2
     us[1] = 1e3 * tot[1] / per;  // D^dagger with the r update, x deferred 3 of 4 times
     return LMX_OK;
 }

@@ -10,6 +10,9 @@ LMX_LO_ALL(LMX_EXTERN)
 #undef LMX_EXTERN
 extern template int launch_lo_loop<SpF32In, SpF32Out, F16Clover>(lmx_context*, const LoArgs&, LinkF16, F16Clover, bool);
 extern template int launch_lo_loop<SpH16In, SpH16Out, F16Clover>(lmx_context*, const LoArgs&, LinkF16, F16Clover, bool);
+#define LMX_EXTERN_LDS(In, Out, Lk, Cl, NR) extern template int launch_lo_lds<In, Out, Lk, Cl>(lmx_context*, const LoArgs&, Lk, Cl, bool);
+LMX_LO_COMBOS(LMX_EXTERN_LDS, 1)
+#undef LMX_EXTERN_LDS
 
 int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
                       double2* out, const double2* in, int nrhs, int flags, const Epilogue& e) {
@@ -53,6 +56,7 @@ static int launch_lo_sp(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clov
     if constexpr (std::is_same<Lk, LinkF16>::value) {
         if ((a.rhs_block & kRhsLoop) && rhsloop_ok(ctx)) return launch_lo_loop<In, Out, Cl>(ctx, a, U, F, clover);
     }
+    if ((a.rhs_block & kLdsStage) && lds_ok(ctx)) return launch_lo_lds<In, Out, Lk, Cl>(ctx, a, U, F, clover);
     switch (a.rhs_block & 15) {
     case 0:
     case 1:

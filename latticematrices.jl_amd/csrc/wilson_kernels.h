@@ -128,6 +128,13 @@ __device__ __forceinline__ void neg_uniform(cx<R> h[6], bool neg) {
     }
 }
 
+// keep a carried half-spinor in registers: loads after this are not hoisted above it
+template <typename R>
+__device__ __forceinline__ void pin6f(cx<R> (&a)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) asm volatile("" : "+v"(a[k].re), "+v"(a[k].im)::"memory");
+}
+
 template <int MU, int PM, bool FWD, int NR, bool GP, class In, class Lk, typename R>
 __device__ __forceinline__ void hop_half(cx<R> (&acc)[NR][12], const In& in, const Lk& U, const Geom& g, int s,
                                          const int x[4], const int (&rr)[NR]) {
@@ -181,10 +188,14 @@ enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
 #endif
 
 // Fused epilogues: EPI_PAP   Re <aux, out> (the CG's p.Ap on D^dagger, FP64 CG);
-//                  EPI_NORM  |out|^2 (p.Ap = |D p|^2 on D, mixed solver);
+//                  EPI_NORM  |out|^2 (p.Ap = |D p|^2 on D; not instantiated
+//                            for the low-precision launches, which use EPI_TP);
 //                  EPI_XR    out = Ap is not stored: r -= alpha Ap, x += alpha p
-//                            (p = aux) and |r|^2 (mixed solver, on D^dagger).
-enum { EPI_NONE = 0, EPI_PAP = 1, EPI_NORM = 2, EPI_XR = 3 };
+//                            (p = aux) and |r|^2 (mixed solver, on D^dagger);
+//                  EPI_TP    the stencil input is r: out = t = D r + beta t in
+//                            place (= D p by linearity), p = r + beta p_prev
+//                            written at the site, and |t|^2 (mixed solver, on D).
+enum { EPI_NONE = 0, EPI_PAP = 1, EPI_NORM = 2, EPI_XR = 3, EPI_TP = 4 };
 
 struct NoEx {};
 // Extra fields of EPI_XR, all offset to the launch's first RHS.  The x
@@ -211,10 +222,28 @@ struct XrEx {
     int apply_x;
 };
 
+// Extra fields of EPI_TP, offset to the launch's first RHS.  The search
+// direction update p = r + beta p_prev of the conjugate-gradient iteration is
+// folded into the D launch that stencils r: by linearity D p = D r + beta
+// D p_prev, and D p_prev is the previous iteration's t, read and overwritten
+// at the same site.  Both are read on every iteration: on the first
+// (beta == 0) they must hold finite values (the solver's init writes t = 0
+// and p_prev = r).
+template <class In, class Out>
+struct TpEx {
+    In tin;               // t of the previous iteration (the launch's out buffer)
+    In pin;               // p_prev
+    Out pout;             // p, written for live RHS
+    const double* beta;   // beta of RHS r at beta[r * astride]
+    int astride;
+    const int* done;      // retired RHS (p left as it is) at done[r * dstride]
+    int dstride;
+};
+
 // Finish one site from its own spinor psi: out = psi - kappa*hop (+ clover),
 // or 1/2 hop; then the fused epilogue, returning the site's contribution to
 // its reduction.
-template <int MODE, int EPI, class Out, class Cl, class Aux, class Ex, typename R>
+template <int MODE, int EPI, bool SPLIT = false, class Out, class Cl, class Aux, class Ex, typename R>
 __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)[12], const Cl& F, const Aux& aux,
                                               const Ex& ex, int r, int s, const Geom& g, R kappa, R ccoef,
                                               const cx<R> (&acc)[12], bool store) {
@@ -233,6 +262,45 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
         if constexpr (MODE == MODE_CLOVER) clover_add<R>(F, s, g.V, psi, ccoef, res);
     }
     double red = 0.0;
+    if constexpr (EPI == EPI_TP) {
+        // site scales (1 for unscaled formats) are folded into beta.  SPLIT:
+        // p first and a compiler barrier, so that psi is dead before t is
+        // read (the register allocation the LDS-staged kernel needs; the
+        // streaming kernel allocates better with both loads together)
+        const R be = R(ex.beta[(long)r * ex.astride]);
+        if constexpr (SPLIT) {
+            {
+                cx<R> pv[12];
+                R cp;
+                ex.pin.ld12s(r, s, pv, cp);
+                const R bp = be * cp;
+#pragma unroll
+                for (int k = 0; k < 12; ++k)
+                    pv[k] = mk<R>(fmaR(bp, pv[k].re, psi[k].re), fmaR(bp, pv[k].im, psi[k].im));
+                if (store && ex.done[(long)r * ex.dstride] == 0) ex.pout.st12(r, s, pv);
+            }
+            asm volatile("" ::: "memory");
+            cx<R> tv[12];
+            R ct;
+            ex.tin.ld12s(r, s, tv, ct);
+            const R bt = be * ct;
+#pragma unroll
+            for (int k = 0; k < 12; ++k)
+                res[k] = mk<R>(fmaR(bt, tv[k].re, res[k].re), fmaR(bt, tv[k].im, res[k].im));
+        } else {
+            cx<R> pv[12], tv[12];
+            R ct, cp;
+            ex.tin.ld12s(r, s, tv, ct);
+            ex.pin.ld12s(r, s, pv, cp);
+            const R bt = be * ct, bp = be * cp;
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                res[k] = mk<R>(fmaR(bt, tv[k].re, res[k].re), fmaR(bt, tv[k].im, res[k].im));
+                pv[k] = mk<R>(fmaR(bp, pv[k].re, psi[k].re), fmaR(bp, pv[k].im, psi[k].im));
+            }
+            if (store && ex.done[(long)r * ex.dstride] == 0) ex.pout.st12(r, s, pv);
+        }
+    }
     if constexpr (EPI == EPI_XR) {
         const R a = R(ex.alpha[(long)r * ex.astride]);
         cx<R> rv[12];
@@ -282,7 +350,7 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
             red = red + t.re;
         }
     }
-    if constexpr (EPI == EPI_NORM) {
+    if constexpr (EPI == EPI_NORM || EPI == EPI_TP) {
         R site = R(0);  // per-site sum in working precision, accumulated over sites in FP64
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
@@ -517,6 +585,185 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
             for (int q = 1; q < NR; ++q)
                 if (r == q) rsel = rr[q];
             if (r < nr) epi.partials[(long)rsel * epi.pstride + b3 + nblk3 * (t0 + stp)] = tot;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged streaming kernel (low precision, one RHS per work item).  The
+// workgroup walks TL t-slices of its 256-site block (whole x-rows of one
+// z-plane) like k_stream, but keeps the block's spinors at slices t and t+1
+// in LDS: each step stages slice t+1 (one coalesced load per site), then the
+// x neighbours, the y neighbours inside the block, the t+1 neighbour and the
+// site's own spinor are read from LDS; z neighbours and y neighbours across
+// the block edge come from memory, as do the links.  The arithmetic and the
+// hop order are k_stream's, so its results and block partials are
+// bit-identical to k_stream at one RHS per work item.
+// ---------------------------------------------------------------------------
+template <int MU, int PM, bool FWD, typename R>
+__device__ __forceinline__ void hop_given(cx<R> (&acc)[12], cx<R> (&psi)[12], R c, const cx<R> (&u)[9]) {
+    cx<R> h[6], o[6];
+    half_project<PM, MU>(psi, h);
+    if (FWD)
+        matvec_fwd(u, h, o);
+    else
+        matvec_bwd(u, h, o);
+    reconstruct_add_s<PM, MU>(acc, o, c);
+    pin_acc(acc);  // as hop_half: the next hop's loads are not hoisted above this one
+}
+
+template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int FPM, int MODE, int EPI>
+__global__ void __launch_bounds__(256) k_lds(Out out, In in, Lk U, Cl F, Aux aux, Ex ex, Geom g, double kappa_d,
+                                             double ccoef_d, int nrhs, int TL, int ZS, EpiArgs epi) {
+    using R = float;
+    if (epi.gate != nullptr && *epi.gate == 0) return;
+    __shared__ typename In::Slot slot[2];
+    __shared__ double wpart[EPI != EPI_NONE ? kMaxTL : 1][4];
+    const int V3 = g.stride[3];
+    const int nblk3 = V3 >> 8;
+    const int ntc = g.L[3] / TL;
+    const int per = nblk3 / ZS;
+    int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int r = lb % nrhs;
+    lb /= nrhs;
+    const int b3lo = lb % per;
+    lb /= per;
+    const int tc = lb % ntc;
+    const int b3 = (lb / ntc) * per + b3lo;
+    const int tid = threadIdx.x;
+    const int s3 = (b3 << 8) + tid;
+    const int L0 = g.L[0];
+    int x[4];
+    x[0] = s3 % L0;
+    x[1] = (s3 / L0) % g.L[1];
+    x[2] = s3 / g.stride[2];
+    // lanes of the x neighbours (x wraps inside the row) and whether the y
+    // neighbours lie in the block (rows 256 / L0 of it)
+    const int row0 = tid - x[0];
+    const int lxf = row0 + (x[0] + 1 == L0 ? 0 : x[0] + 1);
+    const int lxb = row0 + (x[0] == 0 ? L0 - 1 : x[0] - 1);
+    const int yrow = tid / L0;
+    const bool yf_in = yrow + 1 < 256 / L0, yb_in = yrow > 0;
+    const int t0 = tc * TL;
+    const R kappa = R(kappa_d), ccoef = R(ccoef_d);
+    // carried backward t-hop of the current slice: B = U_t(x-t)^dag P psi(x-t)
+    cx<R> B[6];
+    {
+        x[3] = t0;
+        const int s = s3 + V3 * t0;
+        bool w;
+        const int tm = nbr_bwd(s, x, 3, g, w);
+        const bool ng = w && g.phase_neg[3];
+        cx<R> u[9], psi[12], h[6];
+        U.template ld9<R>(3, tm, u);
+        in.ld12(r, tm, psi);
+        half_project<-FPM, 3>(psi, h);
+        if constexpr (MODE == MODE_CLOVER)
+            neg_select(h, ng);
+        else
+            neg_uniform(h, ng);
+        matvec_bwd(u, h, B);
+    }
+    in.stage(r, s3 + V3 * t0, slot[0], tid);
+    for (int stp = 0; stp < TL; ++stp) {
+        const int t = t0 + stp;
+        x[3] = t;
+        const int s = s3 + V3 * t;
+        const int cur = stp & 1;
+        bool wt;
+        const int tp = nbr_fwd(s, x, 3, g, wt);
+        in.stage(r, tp, slot[cur ^ 1], tid);
+        __syncthreads();
+        const typename In::Slot& sc = slot[cur];
+        cx<R> acc[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[k] = mk<R>(R(0), R(0));
+        {  // x forward / backward: both neighbours in the block
+            cx<R> u[9], psi[12];
+            R c;
+            U.template ld9<R>(0, s, u);
+            In::lds_ld12s(sc, lxf, psi, c);
+            hop_given<0, FPM, true>(acc, psi, c, u);
+            bool w;
+            const int nb = nbr_bwd(s, x, 0, g, w);
+            U.template ld9<R>(0, nb, u);
+            In::lds_ld12s(sc, lxb, psi, c);
+            hop_given<0, -FPM, false>(acc, psi, c, u);
+        }
+        {  // y: from the block unless across its edge
+            cx<R> u[9], psi[12];
+            R c;
+            bool w;
+            U.template ld9<R>(1, s, u);
+            const int nf = nbr_fwd(s, x, 1, g, w);
+            if (yf_in)
+                In::lds_ld12s(sc, tid + L0, psi, c);
+            else
+                in.ld12s(r, nf, psi, c);
+            hop_given<1, FPM, true>(acc, psi, c, u);
+            const int nb = nbr_bwd(s, x, 1, g, w);
+            U.template ld9<R>(1, nb, u);
+            if (yb_in)
+                In::lds_ld12s(sc, tid - L0, psi, c);
+            else
+                in.ld12s(r, nb, psi, c);
+            hop_given<1, -FPM, false>(acc, psi, c, u);
+        }
+        {  // z: from memory
+            cx<R> u[9], psi[12];
+            R c;
+            bool w;
+            U.template ld9<R>(2, s, u);
+            in.ld12s(r, nbr_fwd(s, x, 2, g, w), psi, c);
+            hop_given<2, FPM, true>(acc, psi, c, u);
+            const int nb = nbr_bwd(s, x, 2, g, w);
+            U.template ld9<R>(2, nb, u);
+            in.ld12s(r, nb, psi, c);
+            hop_given<2, -FPM, false>(acc, psi, c, u);
+        }
+        // t forward from the staged slice t+1, t backward from the carried B
+        cx<R> ut[9];
+        U.template ld9<R>(3, s, ut);
+        {
+            const bool ng = wt && g.phase_neg[3];
+            cx<R> psi[12], h[6], o[6];
+            R c;
+            In::lds_ld12s(slot[cur ^ 1], tid, psi, c);
+            half_project<FPM, 3>(psi, h);
+            if constexpr (MODE == MODE_CLOVER)
+                neg_select(h, ng);
+            else
+                neg_uniform(h, ng);
+            matvec_fwd(ut, h, o);
+            reconstruct_add_s<FPM, 3>(acc, o, c);
+            reconstruct_add<-FPM, 3>(acc, B);
+            pin_acc(acc);
+        }
+        double red;
+        {
+            cx<R> p[12], h[6];
+            R c;
+            In::lds_ld12s(sc, tid, p, c);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) p[k] = mk<R>(p[k].re * c, p[k].im * c);
+            // the next slice's carried hop first (U_t(x) is then dead in the epilogue)
+            half_project<-FPM, 3>(p, h);
+            matvec_bwd(ut, h, B);
+            pin6f(B);
+            red = finish_site<MODE, EPI, true>(out, p, F, aux, ex, r, s, g, kappa, ccoef, acc, true);
+        }
+        if constexpr (EPI != EPI_NONE) {
+            const double v = wave_halve(red);
+            if ((tid & 63) == 0) wpart[stp][tid >> 6] = v;
+        }
+        __syncthreads();  // slot cur is restaged at the next step
+    }
+    if constexpr (EPI != EPI_NONE) {
+        // one partial per step: the 256 lexicographic sites b3 + nblk3*t,
+        // combined in block_tree's order ((w0 + w2) + (w1 + w3))
+        if (tid < TL) {
+            const double* w = wpart[tid];
+            epi.partials[(long)r * epi.pstride + b3 + nblk3 * (t0 + tid)] = (w[0] + w[2]) + (w[1] + w[3]);
         }
     }
 }
@@ -936,7 +1183,7 @@ inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks, int
 // instantiates the per-site kernel for +-1 phases too (else the GP kernel
 // covers lattices whose 3-D volume is not a multiple of 256).  The fused
 // epilogues exist where D^dagger D uses them: EPI_PAP and EPI_XR on
-// D^dagger, EPI_NORM on D.
+// D^dagger, EPI_NORM and EPI_TP on D.
 template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int NR, int EPI, bool SITE_NG, bool DONLY_OK>
 int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex, double kappa, double ccoef,
              int ngroups, int nrhs, int flags, bool clover, const EpiArgs& epi, int min_blocks, int tl_hint = 0,
@@ -952,7 +1199,8 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
     hipStream_t st = ctx->stream;
     // FP64 streaming Wilson / hopping-only launches use the two-lanes-per-site kernel
     constexpr bool kTwoLane = LMX_TWO_LANE && std::is_same<In, SpF64In>::value && NR == 1;
-    if (EPI != EPI_NONE && (donly || dag != (EPI != EPI_NORM))) {
+    constexpr bool kOnD = EPI == EPI_NORM || EPI == EPI_TP;
+    if (EPI != EPI_NONE && (donly || dag == kOnD)) {
         set_error("fused reduction %d not built for this operator (p.Ap and x/r updates on D^dagger, |D p|^2 on D)",
                   EPI);
         return LMX_ERR_UNSUPPORTED;
@@ -986,7 +1234,7 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
             LMX_GO(1, MODE_CLOVER);
         else
             LMX_GO(1, MODE_WILSON);
-    } else if constexpr (EPI == EPI_NORM) {
+    } else if constexpr (kOnD) {
         if (clover)
             LMX_GO(-1, MODE_CLOVER);
         else
@@ -1036,6 +1284,18 @@ XrEx<In, Out, APPLY> make_xr(const LoArgs& a, int V, int r0) {
     return ex;
 }
 
+// The EPI_TP fields of a launch starting at RHS r0 (t is the launch's out).
+template <class In, class Out>
+TpEx<In, Out> make_tp(const LoArgs& a, int V, int r0) {
+    return TpEx<In, Out>{In::at(a.out, a.out_norm, V, r0),
+                         In::at(a.pprev, a.pprev_norm, V, r0),
+                         Out::at(a.pnext, a.pnext_norm, V, r0),
+                         a.beta + (long)r0 * a.astride,
+                         a.astride,
+                         a.done + (long)r0 * a.dstride,
+                         a.dstride};
+}
+
 // One low-precision (In, Out, Lk, Cl, NR) combination: (rhs_block >> 4) & 0xff
 // right-hand sides per launch, NR per work item, (rhs_block >> 12) & 0xff
 // t-slices per streaming workgroup, (rhs_block >> 20) & 0xf z-ranges (the
@@ -1067,9 +1327,10 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
                 LMX_TRY((dispatch<In, Lk, Out, Cl, In, XrEx<In, Out, 0>, NR, EPI_XR, false, false>(
                     ctx, o, i, U, F, p, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
             }
-        } else if (a.epi == EPI_NORM) {
-            LMX_TRY((dispatch<In, Lk, Out, Cl, In, NoEx, NR, EPI_NORM, false, false>(
-                ctx, o, i, U, F, i, none, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
+        } else if (a.epi == EPI_TP) {
+            const TpEx<In, Out> ex = make_tp<In, Out>(a, V, r0);
+            LMX_TRY((dispatch<In, Lk, Out, Cl, In, TpEx<In, Out>, NR, EPI_TP, false, false>(
+                ctx, o, i, U, F, i, ex, a.kappa, ccoef, ngroups, n, a.flags, clover, epi, 512, tl, zs)));
         } else if (a.epi == EPI_NONE) {
             // the single-precision user apply (FP32 spinors and links) also
             // serves the hopping-only operator
@@ -1124,22 +1385,79 @@ int launch_lo_loop(lmx_context* ctx, const LoArgs& a, LinkF16 U, Cl F, bool clov
             else
                 LMX_LOOP_XR(0);
 #undef LMX_LOOP_XR
-        } else if (a.epi == EPI_NORM) {
+        } else if (a.epi == EPI_TP) {
             if (dag) {
-                set_error("|D p|^2 runs on D");
+                set_error("the direction update runs on D");
                 return LMX_ERR_UNSUPPORTED;
             }
-            const NoEx none{};
+            const TpEx<In, Out> ex = make_tp<In, Out>(a, V, r0);
             if (clover)
-                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, NoEx, -1, MODE_CLOVER, EPI_NORM>), dim3(nb), dim3(256),
-                                   0, st, o, i, U, F, i, none, ctx->g, a.kappa, ccoef, n, epi);
+                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, TpEx<In, Out>, -1, MODE_CLOVER, EPI_TP>), dim3(nb),
+                                   dim3(256), 0, st, o, i, U, F, i, ex, ctx->g, a.kappa, ccoef, n, epi);
             else
-                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, NoEx, -1, MODE_WILSON, EPI_NORM>), dim3(nb), dim3(256),
-                                   0, st, o, i, U, F, i, none, ctx->g, a.kappa, ccoef, n, epi);
+                hipLaunchKernelGGL((k_rhsloop<In, Out, Cl, In, TpEx<In, Out>, -1, MODE_WILSON, EPI_TP>), dim3(nb),
+                                   dim3(256), 0, st, o, i, U, F, i, ex, ctx->g, a.kappa, ccoef, n, epi);
         } else {
             set_error("the RHS-loop shape is built for the mixed solver's inner stencils");
             return LMX_ERR_UNSUPPORTED;
         }
+        LMX_HIP(hipGetLastError());
+    }
+    return LMX_OK;
+}
+
+// The LDS-staged shape (rhs_block bit kLdsStage): one RHS per work item,
+// (rhs_block >> 4) & 0xff RHS per launch, t-slices per workgroup and z-ranges
+// as for launch_lo_nr; built for the mixed solver's two inner stencils (D with
+// the direction update, D^dagger with the r / x update) on lattices that pass
+// lds_ok.
+template <class In, class Out, class Lk, class Cl>
+int launch_lo_lds(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
+    const double ccoef = (-a.kappa) * a.csw;
+    int per_launch = (a.rhs_block >> 4) & 0xff;
+    if (per_launch <= 0) per_launch = a.nrhs;
+    const int tl = (a.rhs_block >> 12) & 0xff;
+    const int zs = (a.rhs_block >> 20) & 0xf;
+    const int V = (int)ctx->V;
+    const bool dag = (a.flags & LMX_DAGGER) != 0;
+    hipStream_t st = ctx->stream;
+    if (!((a.epi == EPI_TP && !dag) || (a.epi == EPI_XR && dag))) {
+        set_error("the LDS-staged shape is built for the mixed solver's inner stencils");
+        return LMX_ERR_UNSUPPORTED;
+    }
+    for (int r0 = 0; r0 < a.nrhs; r0 += per_launch) {
+        const int n = std::min(per_launch, a.nrhs - r0);
+        const Shape sh = pick_shape(ctx, n, 512, tl, zs);
+        const int nb = (int)(ctx->V / 256 / sh.TL) * n;
+        Out o = Out::at(a.out, a.out_norm, V, r0);
+        In i = In::at(a.in, a.in_norm, V, r0);
+        EpiArgs epi{a.partials ? a.partials + (long)r0 * a.pstride : nullptr, a.pstride, nullptr, a.gate};
+#define LMX_LDS_GO(EX, FPM, EPI)                                                                                 \
+    do {                                                                                                        \
+        if (clover)                                                                                             \
+            hipLaunchKernelGGL((k_lds<In, Lk, Out, Cl, In, EX, FPM, MODE_CLOVER, EPI>), dim3(nb), dim3(256), 0, st, \
+                               o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, sh.TL, sh.ZS, epi);                \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_lds<In, Lk, Out, Cl, In, EX, FPM, MODE_WILSON, EPI>), dim3(nb), dim3(256), 0, st, \
+                               o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, sh.TL, sh.ZS, epi);                \
+    } while (0)
+        if (a.epi == EPI_TP) {
+            using EX = TpEx<In, Out>;
+            const EX ex = make_tp<In, Out>(a, V, r0);
+            const In p = i;
+            LMX_LDS_GO(EX, -1, EPI_TP);
+        } else if (a.apply_x) {
+            using EX = XrEx<In, Out, 1>;
+            const EX ex = make_xr<In, Out, 1>(a, V, r0);
+            const In p = In::at(a.aux, a.aux_norm, V, r0);
+            LMX_LDS_GO(EX, 1, EPI_XR);
+        } else {
+            using EX = XrEx<In, Out, 0>;
+            const EX ex = make_xr<In, Out, 0>(a, V, r0);
+            const In p = In::at(a.aux, a.aux_norm, V, r0);
+            LMX_LDS_GO(EX, 1, EPI_XR);
+        }
+#undef LMX_LDS_GO
         LMX_HIP(hipGetLastError());
     }
     return LMX_OK;

@@ -19,7 +19,19 @@ LMX_LO_COMBO_1(LMX_INST, LMX_LO_NR)
 LMX_LO_COMBO_2(LMX_INST, LMX_LO_NR)
 #endif
 #undef LMX_INST
-// the RHS-loop shape (FP16 links), built with the one-RHS-per-work-item objects
+// the LDS-staged and RHS-loop (FP16 links) shapes, built with the
+// one-RHS-per-work-item objects
+#if LMX_LO_NR == 1
+#define LMX_INST_LDS(In, Out, Lk, Cl, NR) template int launch_lo_lds<In, Out, Lk, Cl>(lmx_context*, const LoArgs&, Lk, Cl, bool);
+#if LMX_LO_COMBO == 0
+LMX_LO_COMBO_0(LMX_INST_LDS, 1)
+#elif LMX_LO_COMBO == 1
+LMX_LO_COMBO_1(LMX_INST_LDS, 1)
+#else
+LMX_LO_COMBO_2(LMX_INST_LDS, 1)
+#endif
+#undef LMX_INST_LDS
+#endif
 #if LMX_LO_NR == 1 && LMX_LO_COMBO == 1
 template int launch_lo_loop<SpF32In, SpF32Out, F16Clover>(lmx_context*, const LoArgs&, LinkF16, F16Clover, bool);
 #elif LMX_LO_NR == 1 && LMX_LO_COMBO == 2

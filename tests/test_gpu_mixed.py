@@ -307,3 +307,32 @@ def test_rhs_loop_shape(gpu, precision, per, csw):
     # the inner kernels alone at this shape: timed without error
     t = lmx.inner_kernel_times(D, precision, 5, RHS_LOOP | (per << 4) | 1, groups=1)
     assert t["D"] > 0 and t["Ddag"] > 0
+
+
+LDS_STAGE = 1 << 29
+
+
+@pytest.mark.parametrize("precision,csw", [("h16", 0.0), ("h16", 1.0), ("f16", 0.0), ("f32", 1.0)])
+def test_lds_stage_shape(gpu, precision, csw):
+    # the LDS-staged stencils (x, in-block y, t+1 and own spinors from LDS)
+    # repeat the streaming kernel's arithmetic, hop order and block partials,
+    # so the whole solve is bit-identical to the streaming shape at one RHS per
+    # work item; 32x16x2x8 holds two 256-site blocks per z-plane, so y
+    # neighbours are read both inside the block and across its edge, and 3
+    # RHS in launches of 2 leave a ragged last launch
+    L, kappa, rtol = (32, 16, 2, 8), 0.124, 1e-10
+    U, b, lat, Ug, D = setup(L, "weak", 331, kappa, csw=csw, nrhs=3)
+    print(f"LDS-staged {precision} csw={csw} L={L}", flush=True)
+    bg = lat.spinor_from_numpy(b)
+    shape = (4 << 12) | (2 << 4) | 1
+    x, xs = lat.spinor(3), lat.spinor(3)
+    res = lmx.cg_batched_(x, D, bg, precision=precision, rtol=rtol, use_graph=True, rhs_block=LDS_STAGE | shape)
+    ref = lmx.cg_batched_(xs, D, bg, precision=precision, rtol=rtol, use_graph=True, rhs_block=shape)
+    assert [r.iterations for r in res] == [q.iterations for q in ref]
+    assert np.array_equal(x.to_numpy(), xs.to_numpy())
+    checks = oracle_check(U, L, kappa, b, x.to_numpy(), rtol, csw=csw)
+    for r, (tr, err) in zip(res, checks):
+        print(f"  iters={r.iterations} true_rel={tr:.3e} err_vs_fp64={err:.3e}", flush=True)
+        assert r.converged and tr <= rtol * 1.05 and err <= 1e-7
+    t = lmx.inner_kernel_times(D, precision, 3, LDS_STAGE | shape, groups=1)
+    assert t["D"] > 0 and t["Ddag"] > 0
