@@ -325,7 +325,7 @@ def main():
             f"{a['true_relative_residual_max']:.3e}; RHS0 error vs FP64 {erra:.3e}")
 
     traffic, traffic_src = measured_traffic("k_stream_fp64_dw_32x32x32x32", L)
-    pair_name = f"k_stream_{args.precision}_pair_32x32x32x32_{n}rhs"
+    pair_name = f"k_lds_{args.precision}_pair_32x32x32x32_{n}rhs"
     tr_D, tr_src = measured_traffic(pair_name, L, "D_traffic_bytes_per_launch", n)
     tr_X, _ = measured_traffic(pair_name, L, "Ddag_traffic_bytes_per_launch", n)
     cpu = None
@@ -344,10 +344,12 @@ def main():
 
     lo_name = {"h16": "SpH16In, LinkF16, SpH16Out", "f16": "SpF32In, LinkF16, SpF32Out",
                "f32": "SpF32In, LinkF32, SpF32Out"}[args.precision]
-    roof_X = roofline(f"k_stream<{lo_name}, ..., XrEx, +1, 0, NR, 3> (D^dagger with the r / x update, {n} RHS, "
-                      f"the dominant kernel of the timed solve)", bX * V, kt["Ddag"], tr_X, tr_src)
-    roof_D = roofline(f"k_stream<{lo_name}, ..., TpEx, -1, 0, NR, 4> (D of r with the direction update and "
-                      f"|Dp|^2, {n} RHS)", bD * V, kt["D"], tr_D, tr_src)
+    fam = "k_lds" if blk & (1 << 29) else ("k_rhsloop" if blk & (1 << 28) else "k_stream")
+    roof_X = roofline(f"{fam}<{lo_name}, ..., XrEx, +1, ...> (D^dagger with the r / x update, {n} RHS, the mean "
+                      f"of the deferring and x-applying launches as the solver runs them; the dominant kernel of "
+                      f"the timed solve)", bX * V, kt["Ddag"], tr_X, tr_src)
+    roof_D = roofline(f"{fam}<{lo_name}, ..., TpEx, -1, ...> (D of r with the direction update and |Dp|^2, "
+                      f"{n} RHS)", bD * V, kt["D"], tr_D, tr_src)
     roof_64 = roofline("k_stream<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, NoEx, -1, 0, 1, 0, false> "
                        "(FP64 D_W, 32^4; the reliable updates and the FP64 CG)", bytes_d64, t_d64 * 1e3, traffic,
                        traffic_src)

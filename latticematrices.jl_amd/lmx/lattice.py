@@ -812,8 +812,10 @@ def cg_batched_(x: SpinorField, D: WilsonDiracOperator4D, b: SpinorField, precis
     reliable updates, FP64 reductions and FP64 solution accumulation, and
     stop on the FP64 true residual.  x holds the initial guess.  rhs_block
     encodes (z-ranges) << 20 | (t-slices per workgroup) << 12 | (RHS per
-    launch) << 4 | (RHS per work item, 1, 2 or 4), zero fields automatic; 0
-    uses the autotuned or default shape.  Returns one CGResult per RHS."""
+    launch) << 4 | (RHS per work item, 1, 2 or 4), zero fields automatic;
+    bit 29 selects the LDS-staged stencils (one RHS per work item), bit 28
+    the RHS-loop stencils (FP16 links, RHS per launch only); 0 uses the
+    autotuned or default shape.  Returns one CGResult per RHS."""
     if x.nrhs != b.nrhs:
         raise ArgumentError(_lib.LMX_ERR_ARG, "x and b must hold the same number of right-hand sides")
     if x.data.data_ptr() == b.data.data_ptr():
