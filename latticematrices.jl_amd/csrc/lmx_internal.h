@@ -142,15 +142,12 @@ inline bool rhsloop_ok(const lmx_context* ctx) {
     return V3 % 256 == 0 && !g.general && g.phase_one[0] && g.phase_one[1] && g.phase_one[2];
 }
 // rhs_block bit of the LDS-staged streaming shape: one RHS per work item,
-// the workgroup's 256-site block at slices t and t+1 held in LDS, so that x,
-// in-block y and t neighbours and the site's own spinor are read from LDS.
+// the workgroup's 256-site block at slices t and t+1 held in LDS, so that the
+// x, y, z neighbours inside the block, the t+1 neighbour and the site's own
+// spinor are read from LDS.
 constexpr int kLdsStage = 1 << 29;
-// Whether a launch can take it: the RHS-loop conditions, and blocks of whole
-// x-rows inside one z-plane (L0 divides 256, 256 / L0 divides L1).
-inline bool lds_ok(const lmx_context* ctx) {
-    const Geom& g = ctx->g;
-    return rhsloop_ok(ctx) && g.L[0] <= 256 && 256 % g.L[0] == 0 && g.L[1] % (256 / g.L[0]) == 0;
-}
+// Whether a launch can take it: the RHS-loop conditions.
+inline bool lds_ok(const lmx_context* ctx) { return rhsloop_ok(ctx); }
 struct LoArgs {
     int link_prec, spinor_prec;
     const void* links;

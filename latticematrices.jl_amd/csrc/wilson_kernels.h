@@ -591,12 +591,11 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
 
 // ---------------------------------------------------------------------------
 // LDS-staged streaming kernel (low precision, one RHS per work item).  The
-// workgroup walks TL t-slices of its 256-site block (whole x-rows of one
-// z-plane) like k_stream, but keeps the block's spinors at slices t and t+1
-// in LDS: each step stages slice t+1 (one coalesced load per site), then the
-// x neighbours, the y neighbours inside the block, the t+1 neighbour and the
-// site's own spinor are read from LDS; z neighbours and y neighbours across
-// the block edge come from memory, as do the links.  The arithmetic and the
+// workgroup walks TL t-slices of its 256-site block like k_stream, but keeps
+// the block's spinors at slices t and t+1 in LDS: each step stages slice t+1
+// (one coalesced load per site), then every x, y or z neighbour that lies in
+// the block, the t+1 neighbour and the site's own spinor are read from LDS;
+// the other neighbours come from memory, as do the links.  The arithmetic and the
 // hop order are k_stream's, so its results and block partials are
 // bit-identical to k_stream at one RHS per work item.
 // ---------------------------------------------------------------------------
@@ -612,7 +611,35 @@ __device__ __forceinline__ void hop_given(cx<R> (&acc)[12], cx<R> (&psi)[12], R 
     pin_acc(acc);  // as hop_half: the next hop's loads are not hoisted above this one
 }
 
-template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int FPM, int MODE, int EPI>
+// One spatial direction of the LDS-staged kernel: the forward and the
+// backward hop, each neighbour read from the staged 256-site block (sites
+// base .. base + 255 of slice t) when it lies in it, else from memory.
+// WHERE: 0 test each neighbour, 1 always in the block, 2 never in it.
+template <int MU, int FPM, int WHERE, class In, class Lk, typename R>
+__device__ __forceinline__ void hop_pair_lds(cx<R> (&acc)[12], const In& in, const typename In::Slot& sc, const Lk& U,
+                                             const Geom& g, int r, int s, const int x[4], int base) {
+    cx<R> u[9], psi[12];
+    R c;
+    bool w;
+    U.template ld9<R>(MU, s, u);
+    const int nf = nbr_fwd(s, x, MU, g, w);
+    if (WHERE == 1 || (WHERE == 0 && unsigned(nf - base) < 256u))
+        In::lds_ld12s(sc, nf - base, psi, c);
+    else
+        in.ld12s(r, nf, psi, c);
+    hop_given<MU, FPM, true>(acc, psi, c, u);
+    const int nb = nbr_bwd(s, x, MU, g, w);
+    U.template ld9<R>(MU, nb, u);
+    if (WHERE == 1 || (WHERE == 0 && unsigned(nb - base) < 256u))
+        In::lds_ld12s(sc, nb - base, psi, c);
+    else
+        in.ld12s(r, nb, psi, c);
+    hop_given<MU, -FPM, false>(acc, psi, c, u);
+}
+
+// ROWS: the blocks are whole x-rows of one z-plane (L0 divides 256, 256 / L0
+// divides L1), so x neighbours are always in the block and z neighbours never.
+template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int FPM, int MODE, int EPI, bool ROWS>
 __global__ void __launch_bounds__(256) k_lds(Out out, In in, Lk U, Cl F, Aux aux, Ex ex, Geom g, double kappa_d,
                                              double ccoef_d, int nrhs, int TL, int ZS, EpiArgs epi) {
     using R = float;
@@ -637,13 +664,6 @@ __global__ void __launch_bounds__(256) k_lds(Out out, In in, Lk U, Cl F, Aux aux
     x[0] = s3 % L0;
     x[1] = (s3 / L0) % g.L[1];
     x[2] = s3 / g.stride[2];
-    // lanes of the x neighbours (x wraps inside the row) and whether the y
-    // neighbours lie in the block (rows 256 / L0 of it)
-    const int row0 = tid - x[0];
-    const int lxf = row0 + (x[0] + 1 == L0 ? 0 : x[0] + 1);
-    const int lxb = row0 + (x[0] == 0 ? L0 - 1 : x[0] - 1);
-    const int yrow = tid / L0;
-    const bool yf_in = yrow + 1 < 256 / L0, yb_in = yrow > 0;
     const int t0 = tc * TL;
     const R kappa = R(kappa_d), ccoef = R(ccoef_d);
     // carried backward t-hop of the current slice: B = U_t(x-t)^dag P psi(x-t)
@@ -678,49 +698,11 @@ __global__ void __launch_bounds__(256) k_lds(Out out, In in, Lk U, Cl F, Aux aux
         cx<R> acc[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[k] = mk<R>(R(0), R(0));
-        {  // x forward / backward: both neighbours in the block
-            cx<R> u[9], psi[12];
-            R c;
-            U.template ld9<R>(0, s, u);
-            In::lds_ld12s(sc, lxf, psi, c);
-            hop_given<0, FPM, true>(acc, psi, c, u);
-            bool w;
-            const int nb = nbr_bwd(s, x, 0, g, w);
-            U.template ld9<R>(0, nb, u);
-            In::lds_ld12s(sc, lxb, psi, c);
-            hop_given<0, -FPM, false>(acc, psi, c, u);
-        }
-        {  // y: from the block unless across its edge
-            cx<R> u[9], psi[12];
-            R c;
-            bool w;
-            U.template ld9<R>(1, s, u);
-            const int nf = nbr_fwd(s, x, 1, g, w);
-            if (yf_in)
-                In::lds_ld12s(sc, tid + L0, psi, c);
-            else
-                in.ld12s(r, nf, psi, c);
-            hop_given<1, FPM, true>(acc, psi, c, u);
-            const int nb = nbr_bwd(s, x, 1, g, w);
-            U.template ld9<R>(1, nb, u);
-            if (yb_in)
-                In::lds_ld12s(sc, tid - L0, psi, c);
-            else
-                in.ld12s(r, nb, psi, c);
-            hop_given<1, -FPM, false>(acc, psi, c, u);
-        }
-        {  // z: from memory
-            cx<R> u[9], psi[12];
-            R c;
-            bool w;
-            U.template ld9<R>(2, s, u);
-            in.ld12s(r, nbr_fwd(s, x, 2, g, w), psi, c);
-            hop_given<2, FPM, true>(acc, psi, c, u);
-            const int nb = nbr_bwd(s, x, 2, g, w);
-            U.template ld9<R>(2, nb, u);
-            in.ld12s(r, nb, psi, c);
-            hop_given<2, -FPM, false>(acc, psi, c, u);
-        }
+        // x, y, z: each neighbour from the staged block when it lies in it
+        const int base = s - tid;  // the block's first site in slice t
+        hop_pair_lds<0, FPM, ROWS ? 1 : 0>(acc, in, sc, U, g, r, s, x, base);
+        hop_pair_lds<1, FPM, 0>(acc, in, sc, U, g, r, s, x, base);
+        hop_pair_lds<2, FPM, ROWS ? 2 : 0>(acc, in, sc, U, g, r, s, x, base);
         // t forward from the staged slice t+1, t backward from the carried B
         cx<R> ut[9];
         U.template ld9<R>(3, s, ut);
@@ -1421,6 +1403,8 @@ int launch_lo_lds(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
     const int V = (int)ctx->V;
     const bool dag = (a.flags & LMX_DAGGER) != 0;
     hipStream_t st = ctx->stream;
+    const Geom& gg = ctx->g;
+    const bool rows = gg.L[0] <= 256 && 256 % gg.L[0] == 0 && gg.L[1] % (256 / gg.L[0]) == 0;
     if (!((a.epi == EPI_TP && !dag) || (a.epi == EPI_XR && dag))) {
         set_error("the LDS-staged shape is built for the mixed solver's inner stencils");
         return LMX_ERR_UNSUPPORTED;
@@ -1432,14 +1416,19 @@ int launch_lo_lds(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
         Out o = Out::at(a.out, a.out_norm, V, r0);
         In i = In::at(a.in, a.in_norm, V, r0);
         EpiArgs epi{a.partials ? a.partials + (long)r0 * a.pstride : nullptr, a.pstride, nullptr, a.gate};
-#define LMX_LDS_GO(EX, FPM, EPI)                                                                                 \
-    do {                                                                                                        \
-        if (clover)                                                                                             \
-            hipLaunchKernelGGL((k_lds<In, Lk, Out, Cl, In, EX, FPM, MODE_CLOVER, EPI>), dim3(nb), dim3(256), 0, st, \
-                               o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, sh.TL, sh.ZS, epi);                \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_lds<In, Lk, Out, Cl, In, EX, FPM, MODE_WILSON, EPI>), dim3(nb), dim3(256), 0, st, \
-                               o, i, U, F, p, ex, ctx->g, a.kappa, ccoef, n, sh.TL, sh.ZS, epi);                \
+#define LMX_LDS_ONE(EX, FPM, EPI, MODE, ROWS)                                                              \
+    hipLaunchKernelGGL((k_lds<In, Lk, Out, Cl, In, EX, FPM, MODE, EPI, ROWS>), dim3(nb), dim3(256), 0, st, o, i, \
+                       U, F, p, ex, ctx->g, a.kappa, ccoef, n, sh.TL, sh.ZS, epi)
+#define LMX_LDS_GO(EX, FPM, EPI)                           \
+    do {                                                  \
+        if (clover && rows)                               \
+            LMX_LDS_ONE(EX, FPM, EPI, MODE_CLOVER, true);  \
+        else if (clover)                                  \
+            LMX_LDS_ONE(EX, FPM, EPI, MODE_CLOVER, false); \
+        else if (rows)                                    \
+            LMX_LDS_ONE(EX, FPM, EPI, MODE_WILSON, true);  \
+        else                                              \
+            LMX_LDS_ONE(EX, FPM, EPI, MODE_WILSON, false); \
     } while (0)
         if (a.epi == EPI_TP) {
             using EX = TpEx<In, Out>;
@@ -1458,6 +1447,7 @@ int launch_lo_lds(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
             LMX_LDS_GO(EX, 1, EPI_XR);
         }
 #undef LMX_LDS_GO
+#undef LMX_LDS_ONE
         LMX_HIP(hipGetLastError());
     }
     return LMX_OK;

@@ -80,7 +80,7 @@ def test_batch_24x48_spin_colour_plus_uneven(gpu):
         x64 = fp64_solution(D, bg, i)
         errs.append(float(np.linalg.norm(x.to_numpy()[i] - x64) / np.linalg.norm(x64)))
     blk = tune["rhs_block"]
-    print(f"  16 RHS: shape TL={(blk >> 12) & 0xff} ZS={blk >> 20} per={(blk >> 4) & 0xff} nr={blk & 15}, {dt:.2f} s "
+    print(f"  16 RHS: shape TL={(blk >> 12) & 0xff} ZS={(blk >> 20) & 0xf} LDS={(blk >> 29) & 1} per={(blk >> 4) & 0xff} nr={blk & 15}, {dt:.2f} s "
           f"({dt / n:.3f} s/RHS), iterations {[r.iterations for r in res]}, "
           f"updates {[r.reliable_updates for r in res]}, max true rel {max(tr):.3e}, "
           f"error vs FP64 (RHS 0, 12) {errs[0]:.3e} {errs[1]:.3e}", flush=True)
@@ -167,7 +167,7 @@ def test_clover_48x96_mixed_4rhs(gpu):
     x = lat.spinor(4)
     tune = lmx.autotune(D, "h16", 4)
     blk = tune["rhs_block"]
-    print(f"  autotuned shape TL={(blk >> 12) & 0xff} ZS={blk >> 20} per={(blk >> 4) & 0xff} nr={blk & 15}: fused D / D^dagger pair "
+    print(f"  autotuned shape TL={(blk >> 12) & 0xff} ZS={(blk >> 20) & 0xf} LDS={(blk >> 29) & 1} per={(blk >> 4) & 0xff} nr={blk & 15}: fused D / D^dagger pair "
           f"{tune['ms_per_apply'] * 1e3:.0f} us per application per RHS", flush=True)
     # the FP64 clover D_W against the HBM roofline: links 576 + psi 192 +
     # clover cache 432 + out 192 = 1392 algorithmic bytes per site

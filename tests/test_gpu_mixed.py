@@ -312,19 +312,23 @@ def test_rhs_loop_shape(gpu, precision, per, csw):
 LDS_STAGE = 1 << 29
 
 
-@pytest.mark.parametrize("precision,csw", [("h16", 0.0), ("h16", 1.0), ("f16", 0.0), ("f32", 1.0)])
-def test_lds_stage_shape(gpu, precision, csw):
-    # the LDS-staged stencils (x, in-block y, t+1 and own spinors from LDS)
-    # repeat the streaming kernel's arithmetic, hop order and block partials,
-    # so the whole solve is bit-identical to the streaming shape at one RHS per
-    # work item; 32x16x2x8 holds two 256-site blocks per z-plane, so y
-    # neighbours are read both inside the block and across its edge, and 3
-    # RHS in launches of 2 leave a ragged last launch
-    L, kappa, rtol = (32, 16, 2, 8), 0.124, 1e-10
+@pytest.mark.parametrize("precision,csw,L", [("h16", 0.0, (32, 16, 2, 8)), ("h16", 1.0, (32, 16, 2, 8)),
+                                             ("f16", 0.0, (32, 16, 2, 8)), ("f32", 1.0, (32, 16, 2, 8)),
+                                             ("h16", 0.0, (24, 8, 4, 8)), ("f32", 1.0, (8, 8, 8, 4))])
+def test_lds_stage_shape(gpu, precision, csw, L):
+    # the LDS-staged stencils (neighbours inside the workgroup's 256-site
+    # block, the t+1 and own spinors from LDS) repeat the streaming kernel's
+    # arithmetic, hop order and block partials, so the whole solve is
+    # bit-identical to the streaming shape at one RHS per work item.
+    # 32x16x2x8: two blocks per z-plane (y neighbours inside the block and
+    # across its edge); 24x8x4x8: blocks cut x-rows (x neighbours in the next
+    # block); 8x8x8x4: four z-planes per block (z neighbours in the block).
+    # 3 RHS in launches of 2 leave a ragged last launch.
+    kappa, rtol = 0.124, 1e-10
     U, b, lat, Ug, D = setup(L, "weak", 331, kappa, csw=csw, nrhs=3)
     print(f"LDS-staged {precision} csw={csw} L={L}", flush=True)
     bg = lat.spinor_from_numpy(b)
-    shape = (4 << 12) | (2 << 4) | 1
+    shape = (min(4, L[3]) << 12) | (2 << 4) | 1
     x, xs = lat.spinor(3), lat.spinor(3)
     res = lmx.cg_batched_(x, D, bg, precision=precision, rtol=rtol, use_graph=True, rhs_block=LDS_STAGE | shape)
     ref = lmx.cg_batched_(xs, D, bg, precision=precision, rtol=rtol, use_graph=True, rhs_block=shape)
