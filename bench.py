@@ -29,6 +29,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -63,6 +64,8 @@ def parse():
     ap.add_argument("--cpu-maxiter", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--output", type=str, default=None,
+                    help="append the benchmark's result rows (the reference's CSV columns) to this CSV file")
     ap.add_argument("--mode", type=str, default="full", choices=["full", "kernels"],
                     help="kernels: only the dslash / copy kernel timings (for profiler counter runs)")
     return ap.parse_args()
@@ -132,6 +135,46 @@ def copy_calibration(lat):
     ms = timed(lat, lambda: b.copy_(a), 10)
     del a, b
     return ms
+
+
+def result_rows(L, n, args, solves):
+    """The reference benchmark's VERIFY / RESULT lines (benchmark/ddagd_cg/ddagd_cg.jl:883-892) for each
+    timed solve, printed on stderr (stdout keeps the one JSON line), and its CSV row (:835-880, the columns
+    that apply to this build) appended to --output.  solves: (precision label, iterations, [seconds per
+    solve], true relative residual, converged)."""
+    import csv
+    import datetime
+    import torch
+    dev = torch.cuda.get_device_name(0).replace(" ", "_")
+    lattice = "x".join(map(str, L))
+    verify_rtol = 10 * args.rtol
+    rows = []
+    for prec, iters, secs, true_res, conv in solves:
+        med = sorted(secs)[len(secs) // 2]
+        passed = bool(conv) and math.isfinite(true_res) and true_res <= verify_rtol
+        ms_it = 0.0 if iters == 0 else 1e3 * med / iters
+        log(f"VERIFY passed={str(passed).lower()} converged={str(bool(conv)).lower()} "
+            f"true_relative_residual={true_res:.9e} threshold={verify_rtol:.9e}")
+        log(f"RESULT operator=wilson backend=hip device={dev} device_ordinal=0 precision={prec} ranks=1 "
+            f"threads=1 lattice={lattice} grid=1x1x1x1 iterations={iters} median_seconds={med:.9f} "
+            f"ms_per_iteration={ms_it:.9f} true_residual={true_res:.9e} passed={str(passed).lower()}")
+        rows.append({"timestamp": datetime.datetime.now(datetime.timezone.utc).strftime("%Y-%m-%dT%H:%M:%S.%f")[:-3] + "Z",
+                     "operator": "wilson", "backend": "hip", "device": dev, "precision": prec, "mpi_ranks": 1,
+                     "threads_per_rank": 1, "lattice": lattice, "process_grid": "1x1x1x1", "local_lattice": lattice,
+                     "halo": 0, "rtol": args.rtol, "atol": 0.0, "verify_rtol": verify_rtol, "maxiter": 20000,
+                     "repeats": len(secs), "nrhs": n if prec != "Float64" else 1, "iterations": iters,
+                     "converged": bool(conv), "true_relative_residual": true_res, "passed": passed,
+                     "min_solve_seconds": min(secs), "median_solve_seconds": med,
+                     "max_solve_seconds": max(secs), "milliseconds_per_iteration": ms_it,
+                     "torch_version": torch.__version__, "hip_version": str(torch.version.hip)})
+    if args.output:
+        new = not os.path.exists(args.output)
+        with open(args.output, "a", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=list(rows[0].keys()))
+            if new:
+                w.writeheader()
+            w.writerows(rows)
+        log(f"CSV appended: {os.path.abspath(args.output)}")
 
 
 def host_cpu():
@@ -249,14 +292,19 @@ def main():
 
     res64 = solve64()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    t64s = []
     for _ in range(args.fp64_steps):
+        t0 = time.perf_counter()
         res64 = solve64()
-    torch.cuda.synchronize()
-    t64 = (time.perf_counter() - t0) / args.fp64_steps
+        torch.cuda.synchronize()
+        t64s.append(time.perf_counter() - t0)
+    t64 = sum(t64s) / len(t64s)
     log(f"bench: FP64 CG {t64:.3f} s/RHS, {res64.iterations} iterations, converged={res64.converged}")
     require_converged("FP64", [res64], args.rtol)
     x64 = x1.to_numpy()
+    lmx.mul_(Ap, N, x1)  # the FP64 true residual |b - A x| / |b| (true_relative_residual!, ddagd_cg.jl:825)
+    b0n = b0.to_numpy()
+    true64 = float(np.linalg.norm(Ap.to_numpy() - b0n) / np.linalg.norm(b0n))
 
     # ---- mixed multi-RHS CG ------------------------------------------------
     tune = lmx.autotune(D, args.precision, n, None)  # measured in this run, never read from a cache
@@ -282,8 +330,12 @@ def main():
         log(f"bench: warmup {w}: iterations {[q.iterations for q in resm]}")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    tsteps = []
     for k in range(args.steps):
+        tk = time.perf_counter()
         resm = solve_mixed()
+        torch.cuda.synchronize()
+        tsteps.append(time.perf_counter() - tk)
         log(f"bench: step {k}: iterations {[q.iterations for q in resm]} "
             f"updates {[q.reliable_updates for q in resm]}")
     torch.cuda.synchronize()
@@ -295,6 +347,10 @@ def main():
     require_converged(f"mixed {args.precision}", resm, args.rtol)
     log(f"bench: mixed {args.precision} x{n}: {tm:.3f} s/batch = {per_rhs:.4f} s/RHS; speedup vs FP64 "
         f"{t64 / per_rhs:.2f}x; max true rel residual {max(true_res):.3e}; RHS0 error vs FP64 {err0:.3e}")
+    # the reference benchmark's result rows: the FP64 solve, and the mixed batch per RHS
+    result_rows(L, n, args, [("Float64", res64.iterations, t64s, true64, res64.converged),
+                             (f"{args.precision}-mixed-x{n}", max(q.iterations for q in resm),
+                              [t / n for t in tsteps], max(true_res), all(q.converged for q in resm))])
 
     # the configuration's own inner format (FP32 spinors and links) and FP16
     # links with FP32 spinors, each timed once after the headline solve
