@@ -130,6 +130,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 typedef double lmx_dv2 __attribute__((ext_vector_type(2)));
 typedef float lmx_fv2 __attribute__((ext_vector_type(2)));
 typedef int lmx_iv4 __attribute__((ext_vector_type(4)));
+typedef float lmx_fv4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void st_out(double2* p, double re, double im) {
 #if LMX_NT_STORE
     lmx_dv2 v = {re, im};
@@ -156,7 +157,6 @@ __device__ __forceinline__ void st_out(int4* p, int4 w) {
 }
 __device__ __forceinline__ void st_out(float4* p, float4 w) {
 #if LMX_NT_STORE
-    typedef float lmx_fv4 __attribute__((ext_vector_type(4)));
     lmx_fv4 v = {w.x, w.y, w.z, w.w};
     __builtin_nontemporal_store(v, (lmx_fv4*)p);
 #else
@@ -178,6 +178,48 @@ __device__ __forceinline__ void st_out(float* p, float v) {
 template <typename T>
 __device__ __forceinline__ T ldv(const T* ubase, int s) {
     return *(const T*)((const char*)ubase + (unsigned)s * (unsigned)sizeof(T));
+}
+
+// Streamed loads (nontemporal): for fields a launch reads once at the site
+// itself (the fused epilogues' r, t, p, x_inner), so that they do not evict
+// the neighbour spinors and links the stencil re-reads from L2.
+#ifndef LMX_NT_STREAM
+#define LMX_NT_STREAM 1
+#endif
+__device__ __forceinline__ int4 ldv_stream(const int4* ubase, int s) {
+    const int4* a = (const int4*)((const char*)ubase + (unsigned)s * 16u);
+#if LMX_NT_STREAM
+    const lmx_iv4 v = __builtin_nontemporal_load((const lmx_iv4*)a);
+    return make_int4(v.x, v.y, v.z, v.w);
+#else
+    return *a;
+#endif
+}
+__device__ __forceinline__ float ldv_stream(const float* ubase, int s) {
+    const float* a = (const float*)((const char*)ubase + (unsigned)s * 4u);
+#if LMX_NT_STREAM
+    return __builtin_nontemporal_load(a);
+#else
+    return *a;
+#endif
+}
+__device__ __forceinline__ float2 ldv_stream(const float2* ubase, int s) {
+    const float2* a = (const float2*)((const char*)ubase + (unsigned)s * 8u);
+#if LMX_NT_STREAM
+    const lmx_fv2 v = __builtin_nontemporal_load((const lmx_fv2*)a);
+    return make_float2(v.x, v.y);
+#else
+    return *a;
+#endif
+}
+__device__ __forceinline__ float4 ldv_stream(const float4* ubase, int s) {
+    const float4* a = (const float4*)((const char*)ubase + (unsigned)s * 16u);
+#if LMX_NT_STREAM
+    const lmx_fv4 v = __builtin_nontemporal_load((const lmx_fv4*)a);
+    return make_float4(v.x, v.y, v.z, v.w);
+#else
+    return *a;
+#endif
 }
 
 struct SpF64In {
@@ -258,6 +300,20 @@ struct SpF32In {
         float c;
         ld12s(r, s, z, c);
     }
+    // streamed forms (fields read once, at the site itself)
+    __device__ __forceinline__ void ld12s_stream(int r, int s, cx<float>* z, float& c) const {
+        const float2* q = p + (size_t)r * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            float2 v = ldv_stream(q + (size_t)k * V, s);
+            z[k] = mk<float>(v.x, v.y);
+        }
+        c = 1.0f;
+    }
+    __device__ __forceinline__ void ld12_stream(int r, int s, cx<float>* z) const {
+        float c;
+        ld12s_stream(r, s, z, c);
+    }
 };
 struct SpF32Out {
     using R = float;
@@ -286,6 +342,15 @@ struct SpX4In {
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
             float4 v = ldv(q + (size_t)m * V, s);
+            z[2 * m] = mk<float>(v.x, v.y);
+            z[2 * m + 1] = mk<float>(v.z, v.w);
+        }
+    }
+    __device__ __forceinline__ void ld12_stream(int r, int s, cx<float>* z) const {
+        const float4* q = p + (size_t)r * 6 * V;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            float4 v = ldv_stream(q + (size_t)m * V, s);
             z[2 * m] = mk<float>(v.x, v.y);
             z[2 * m + 1] = mk<float>(v.z, v.w);
         }
@@ -363,6 +428,25 @@ struct SpH16In {
     __device__ __forceinline__ void ld12(int r, int s, cx<float>* z) const {
         float c;
         ld12s(r, s, z, c);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) z[k] = mk<float>(z[k].re * c, z[k].im * c);
+    }
+    // streamed forms (fields read once, at the site itself)
+    __device__ __forceinline__ void ld12s_stream(int r, int s, cx<float>* z, float& c) const {
+        const int4* q = p + (size_t)r * 3 * V;
+        c = ldv_stream(scale + (size_t)r * V, s);
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            int4 v = ldv_stream(q + (size_t)g * V, s);
+            z[4 * g + 0] = mk<float>(h16_lo(v.x), h16_hi(v.x));
+            z[4 * g + 1] = mk<float>(h16_lo(v.y), h16_hi(v.y));
+            z[4 * g + 2] = mk<float>(h16_lo(v.z), h16_hi(v.z));
+            z[4 * g + 3] = mk<float>(h16_lo(v.w), h16_hi(v.w));
+        }
+    }
+    __device__ __forceinline__ void ld12_stream(int r, int s, cx<float>* z) const {
+        float c;
+        ld12s_stream(r, s, z, c);
 #pragma unroll
         for (int k = 0; k < 12; ++k) z[k] = mk<float>(z[k].re * c, z[k].im * c);
     }

@@ -272,7 +272,7 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
             {
                 cx<R> pv[12];
                 R cp;
-                ex.pin.ld12s(r, s, pv, cp);
+                ex.pin.ld12s_stream(r, s, pv, cp);
                 const R bp = be * cp;
 #pragma unroll
                 for (int k = 0; k < 12; ++k)
@@ -282,7 +282,7 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
             asm volatile("" ::: "memory");
             cx<R> tv[12];
             R ct;
-            ex.tin.ld12s(r, s, tv, ct);
+            ex.tin.ld12s_stream(r, s, tv, ct);
             const R bt = be * ct;
 #pragma unroll
             for (int k = 0; k < 12; ++k)
@@ -290,8 +290,8 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
         } else {
             cx<R> pv[12], tv[12];
             R ct, cp;
-            ex.tin.ld12s(r, s, tv, ct);
-            ex.pin.ld12s(r, s, pv, cp);
+            ex.tin.ld12s_stream(r, s, tv, ct);
+            ex.pin.ld12s_stream(r, s, pv, cp);
             const R bt = be * ct, bp = be * cp;
 #pragma unroll
             for (int k = 0; k < 12; ++k) {
@@ -304,7 +304,7 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
     if constexpr (EPI == EPI_XR) {
         const R a = R(ex.alpha[(long)r * ex.astride]);
         cx<R> rv[12];
-        ex.rin.ld12(r, s, rv);
+        ex.rin.ld12_stream(r, s, rv);
         R site = R(0);  // per-site sum in working precision, accumulated over sites in FP64
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
@@ -317,15 +317,15 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
         if (live) ex.rout.st12(r, s, rv);
         if (Ex::kApply == 1 && live) {
             cx<R> pv[12], xv[12];
-            aux.ld12(r, s, pv);
-            ex.xin.ld12(r, s, xv);
+            aux.ld12_stream(r, s, pv);
+            ex.xin.ld12_stream(r, s, xv);
             const int m = ex.pmask[(long)r * ex.dstride];
 #pragma unroll
             for (int j = 0; j < kXBatch - 1; ++j) {
                 if (m & (1 << j)) {
                     const R ap = R(ex.alpha_q[(long)r * ex.astride + j]);
                     cx<R> qv[12];
-                    ex.pq[j].ld12(r, s, qv);
+                    ex.pq[j].ld12_stream(r, s, qv);
 #pragma unroll
                     for (int k = 0; k < 12; ++k)
                         xv[k] = mk<R>(fmaR(ap, qv[k].re, xv[k].re), fmaR(ap, qv[k].im, xv[k].im));
