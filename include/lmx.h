@@ -259,6 +259,17 @@ void lmx_cg_params_default(lmx_cg_params* p);
 int lmx_cg_ddagd(lmx_wilson* op, void* x, const void* b, void* r, void* p, void* Ap, void* tmp,
                  const lmx_cg_params* params, lmx_cg_result* result);
 
+/* cg!(x, DdagDOp(D, tmp), b, r, p, Ap; rtol, atol, maxiter) on ComplexF32
+ * fields (src/LinearAlgebras/cg.jl:73-137 is generic in the element type;
+ * single-precision operator tests test/wilson_halfspin.jl:124-130): x, b, r,
+ * p, Ap, tmp are float2 [12][V] fields (lmx_wilson_apply_f32's layout, NC =
+ * 3); the operator runs with FP32 links (and the FP32 clover cache) in FP32
+ * arithmetic, the vector updates in FP32, the reductions' site sums and the
+ * CG scalars in FP64.  The same step order, stopping test and failure
+ * reasons as lmx_cg_ddagd.  Fields must not alias (LMX_ERR_ALIAS). */
+int lmx_cg_ddagd_f32(lmx_wilson* op, void* x, const void* b, void* r, void* p, void* Ap, void* tmp,
+                     const lmx_cg_params* params, lmx_cg_result* result);
+
 /* cg!(x, DdagDOp(D, temp), b, r, p, Ap; rtol, atol, maxiter)
  * (src/LinearAlgebras/cg.jl:73-137) with x and b the LatticeMatrix.A arrays
  * themselves (complex128, column-major [nc,4,L1+2nw,...,L4+2nw]): the solve
@@ -269,6 +280,12 @@ int lmx_cg_ddagd(lmx_wilson* op, void* x, const void* b, void* r, void* p, void*
  * work fields are in `work`).  Synchronous. */
 int lmx_cg_ddagd_padded(lmx_wilson* op, void* x, const void* b, int nw, void* work, size_t work_bytes,
                         const lmx_cg_params* params, lmx_cg_result* result);
+
+/* lmx_cg_ddagd_padded for LatticeMatrix{ComplexF32} fields (complex64
+ * arrays, NC = 3): the solve is lmx_cg_ddagd_f32 on the library layout in
+ * `work` (at least lmx_spinor_bytes(ctx, LMX_F32, 6) bytes).  Synchronous. */
+int lmx_cg_ddagd_padded_f32(lmx_wilson* op, void* x, const void* b, int nw, void* work, size_t work_bytes,
+                            const lmx_cg_params* params, lmx_cg_result* result);
 
 /* No counterpart in the reference: lmx_cg_ddagd for callers that cannot
  * allocate device memory (code between graph captures, tight serving or HMC

@@ -235,6 +235,23 @@ function LatticeMatrices.cg!(x::LM64{NC,nw}, A::DdagDOp{<:WilsonOrClover}, b::LM
     return CGResult(q.converged != 0, Int(q.iterations), q.residual_norm, q.relative_residual, REASONS[q.reason + 1])
 end
 
+# cg! on LatticeMatrix{ComplexF32} (cg! is generic in the element type):
+# FP32 operator and vectors, FP64 reductions, the same workspace
+function LatticeMatrices.cg!(x::LM32{nw}, A::DdagDOp{<:WilsonOrClover}, b::LM32{nw}, r::LM32{nw},
+    p::LM32{nw}, Ap::LM32{nw}; rtol=1e-10, atol=0, maxiter::Integer=5000) where {nw}
+    _validate_cg_arguments(x, A, b, r, p, Ap, rtol, atol, maxiter)
+    op = handle(A.D, b)
+    res = Ref{LmxCGResult}()
+    check(ccall((:lmx_cg_ddagd_padded_f32, liblmx), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Csize_t, Ref{LmxCGParams}, Ref{LmxCGResult}),
+        op.ptr, pointer(x.A), pointer(b.A), nw, pointer(op.work), sizeof(op.work),
+        LmxCGParams(rtol, atol, maxiter), res))
+    mark_halo_dirty!(x)
+    _mark_halo_clean!(x)                  # x's halo was refilled with its phases
+    q = res[]
+    return CGResult(q.converged != 0, Int(q.iterations), q.residual_norm, q.relative_residual, REASONS[q.reason + 1])
+end
+
 # ---------------------------------------------------------------------------
 # Link cotangent and force: the library's own layout in op.work
 # ---------------------------------------------------------------------------

@@ -26,6 +26,8 @@ void clear_error() { g_last_error.clear(); }
 
 int spinor_import(lmx_context* ctx, void* dst, const void* src, int nw, int nrhs);
 int spinor_export(lmx_context* ctx, void* dst, int nw, const void* src, int nrhs);
+int spinor_import_f32(lmx_context* ctx, void* dst, const void* src, int nw, int nrhs);
+int spinor_export_f32(lmx_context* ctx, void* dst, int nw, const void* src, int nrhs);
 int links_import(lmx_context* ctx, void* dst, const void* const src[4], int nw);
 int links_export(lmx_context* ctx, void* const dst[4], int nw, const void* src);
 
@@ -183,6 +185,57 @@ static int cg_ddagd_run(lmx_wilson* op, lmx_context* c, void* x, const void* b, 
             }
             LMX_TRY(launch_cg_update_xr(c, X, R, P, AP, st));
             LMX_TRY(launch_cg_update_p(c, R, P, st));
+        }
+        LMX_TRY(read_state(c, st, c->host_state));
+    }
+    fill_result(*c->host_state, result);
+    return LMX_OK;
+}
+
+// cg! on ComplexF32 fields (float2 [12][V]): the same step order, stopping
+// test and failure reasons as cg_ddagd_run, the vectors and the operator (FP32
+// links, and the FP32 clover cache) in FP32 arithmetic, the reductions summed
+// over sites in FP64 and the CG scalars held in FP64 on the device.
+static int apply_f32_gated(lmx_wilson* op, float2* out, const float2* in, int flags, CGState* st) {
+    LoArgs a;
+    memset(&a, 0, sizeof(a));
+    a.link_prec = LMX_F32;
+    a.spinor_prec = LMX_F32;
+    a.links = op->links32;
+    a.clover = op->clover32;
+    a.kappa = op->kappa;
+    a.csw = op->csw;
+    a.out = out;
+    a.in = in;
+    a.nrhs = 1;
+    a.flags = flags;
+    a.epi = 0;
+    a.rhs_block = (1 << 4) | 1;
+    a.st = st;
+    return launch_wilson_lo(op->ctx, a);
+}
+
+static int cg_ddagd_run_f32(lmx_wilson* op, lmx_context* c, float2* X, const float2* B, float2* R, float2* P,
+                            float2* AP, float2* T, const lmx_cg_params* params, lmx_cg_result* result) {
+    CGState* st = c->state;
+    LMX_TRY(launch_cg_init_state(c, st, params->rtol, params->atol, params->maxiter));
+    LMX_TRY(launch_dot_f32(c, B, B, FIN_BB, st, nullptr));
+    LMX_TRY(apply_f32_gated(op, T, X, 0, st));
+    LMX_TRY(apply_f32_gated(op, AP, T, LMX_DAGGER, st));
+    LMX_TRY(launch_cg_setup_r_f32(c, B, AP, R, P, st));
+    LMX_TRY(read_state(c, st, c->host_state));
+    const int chunk = chunk_of(params);
+    const long max_polls = (long)params->maxiter / chunk + 3;
+    long polls = 0;
+    while (!c->host_state->done) {
+        LMX_CHECK(++polls <= max_polls, LMX_ERR_HIP, "CG state did not advance (iteration %d)",
+                  c->host_state->iteration);
+        for (int k = 0; k < chunk; ++k) {
+            LMX_TRY(apply_f32_gated(op, T, P, 0, st));
+            LMX_TRY(apply_f32_gated(op, AP, T, LMX_DAGGER, st));
+            LMX_TRY(launch_dot_f32(c, P, AP, FIN_PAP, st, nullptr));
+            LMX_TRY(launch_cg_update_xr_f32(c, X, R, P, AP, st));
+            LMX_TRY(launch_cg_update_p_f32(c, R, P, st));
         }
         LMX_TRY(read_state(c, st, c->host_state));
     }
@@ -593,6 +646,45 @@ int lmx_cg_ddagd(lmx_wilson* op, void* x, const void* b, void* r, void* p, void*
     LMX_TRY(validate_cg(f, names, 6, params));
     LMX_CHECK(result != nullptr, LMX_ERR_ARG, "result is NULL");
     return cg_ddagd_run(op, op->ctx, x, b, r, p, Ap, tmp, params, result);
+}
+
+int lmx_cg_ddagd_f32(lmx_wilson* op, void* x, const void* b, void* r, void* p, void* Ap, void* tmp,
+                     const lmx_cg_params* params, lmx_cg_result* result) {
+    clear_error();
+    LMX_TRY(check_op(op));
+    const void* f[6] = {x, b, r, p, Ap, tmp};
+    const char* names[6] = {"x", "b", "r", "p", "Ap", "tmp"};
+    LMX_TRY(validate_cg(f, names, 6, params));
+    LMX_CHECK(result != nullptr, LMX_ERR_ARG, "result is NULL");
+    LMX_TRY(check_nc3(op->ctx, "the single-precision CG"));
+    LMX_TRY(ensure_lo_links(op));
+    return cg_ddagd_run_f32(op, op->ctx, (float2*)x, (const float2*)b, (float2*)r, (float2*)p, (float2*)Ap,
+                            (float2*)tmp, params, result);
+}
+
+int lmx_cg_ddagd_padded_f32(lmx_wilson* op, void* x, const void* b, int nw, void* work, size_t work_bytes,
+                            const lmx_cg_params* params, lmx_cg_result* result) {
+    clear_error();
+    LMX_TRY(check_op(op));
+    const void* f[2] = {x, b};
+    const char* names[2] = {"x", "b"};
+    LMX_TRY(validate_cg(f, names, 2, params));
+    LMX_CHECK(result != nullptr && work != nullptr, LMX_ERR_ARG, "bad cg_ddagd_padded_f32 arguments");
+    lmx_context* c = op->ctx;
+    LMX_TRY(check_nc3(c, "the single-precision CG"));
+    const size_t field = lmx_spinor_bytes(c, LMX_F32, 1);
+    LMX_CHECK(work_bytes >= 6 * field, LMX_ERR_WORKSPACE,
+              "workspace of %zu bytes is below the %zu of six single-precision spinor fields", work_bytes, 6 * field);
+    LMX_TRY(ensure_lo_links(op));
+    unsigned char* w = (unsigned char*)work;
+    float2 *X = (float2*)w, *B = (float2*)(w + field), *R = (float2*)(w + 2 * field), *P = (float2*)(w + 3 * field),
+           *AP = (float2*)(w + 4 * field), *T = (float2*)(w + 5 * field);
+    LMX_TRY(spinor_import_f32(c, X, x, nw, 1));
+    LMX_TRY(spinor_import_f32(c, B, b, nw, 1));
+    LMX_TRY(cg_ddagd_run_f32(op, c, X, B, R, P, AP, T, params, result));
+    LMX_TRY(spinor_export_f32(c, x, nw, X, 1));
+    LMX_HIP(hipStreamSynchronize(c->stream));
+    return LMX_OK;
 }
 
 int lmx_cg_ddagd_noalloc(lmx_wilson* op, void* x, const void* b, void* r, void* p, void* Ap, void* tmp,

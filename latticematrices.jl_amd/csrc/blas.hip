@@ -120,23 +120,40 @@ int launch_finalize(lmx_context* ctx, int nb, int kind, bool want_im, CGState* s
     return LMX_OK;
 }
 
+// The CG vector kernels take fields of element type ComplexF64 (R = double,
+// double2 storage: the bit-exact FP64 CG) or ComplexF32 (R = float, float2:
+// cg! on single-precision fields, which the reference runs in the fields'
+// element type); per-site arithmetic is in R, the site-to-site sums of the
+// reductions in FP64.
+template <typename R>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using T = double2;
+};
+template <>
+struct Vec2<float> {
+    using T = float2;
+};
+
 // dot(A, B) = sum conj(A) B
-template <int NCOMP>
-__global__ void __launch_bounds__(256) k_dot(const double2* __restrict__ A, const double2* __restrict__ B, int V,
-                                             double* partials, int want_im, CGState* st) {
+template <int NCOMP, typename R = double>
+__global__ void __launch_bounds__(256) k_dot(const typename Vec2<R>::T* __restrict__ A,
+                                             const typename Vec2<R>::T* __restrict__ B, int V, double* partials,
+                                             int want_im, CGState* st) {
     if (st != nullptr && st->done) return;
     const int lb = blockIdx.x;
     const int s = lb * 256 + threadIdx.x;
     double re = 0.0, im = 0.0;
     if (s < V) {
-        cx<double> acc = mk<double>(0.0, 0.0);
+        cx<R> acc = mk<R>(R(0), R(0));
 #pragma unroll
         for (int k = 0; k < NCOMP; ++k) {
-            double2 a = A[(long)k * V + s], b = B[(long)k * V + s];
-            acc = cadd(acc, cmul(cconj(mk<double>(a.x, a.y)), mk<double>(b.x, b.y)));
+            const auto a = A[(long)k * V + s], b = B[(long)k * V + s];
+            acc = cadd(acc, cmul(cconj(mk<R>(a.x, a.y)), mk<R>(b.x, b.y)));
         }
-        re = acc.re;
-        im = acc.im;
+        re = double(acc.re);
+        im = double(acc.im);
     }
     store_partials(re, im, want_im != 0, lb, partials);
 }
@@ -158,73 +175,83 @@ __global__ void __launch_bounds__(256) k_scale(double2* __restrict__ out, const 
 }
 
 // r = b - Ap (axpby!(1,b,0,r); axpby!(-1,Ap,1,r)); p = r; rr = |r|^2  (cg.jl:91-95)
-template <int NCOMP>
-__global__ void __launch_bounds__(256) k_cg_setup_r(const double2* __restrict__ b, const double2* __restrict__ Ap,
-                                                    double2* __restrict__ r, double2* __restrict__ p, int V,
-                                                    double* partials, CGState* st) {
+template <int NCOMP, typename R = double>
+__global__ void __launch_bounds__(256) k_cg_setup_r(const typename Vec2<R>::T* __restrict__ b,
+                                                    const typename Vec2<R>::T* __restrict__ Ap,
+                                                    typename Vec2<R>::T* __restrict__ r,
+                                                    typename Vec2<R>::T* __restrict__ p, int V, double* partials,
+                                                    CGState* st) {
     if (st->done) return;
     const int lb = blockIdx.x;
     const int s = lb * 256 + threadIdx.x;
     double re = 0.0;
     if (s < V) {
-        cx<double> acc = mk<double>(0.0, 0.0);
+        cx<R> acc = mk<R>(R(0), R(0));
 #pragma unroll
         for (int k = 0; k < NCOMP; ++k) {
             const long i = (long)k * V + s;
-            double2 bv = b[i], av = Ap[i];
-            double2 rv = make_double2(-1.0 * av.x + bv.x, -1.0 * av.y + bv.y);
+            const auto bv = b[i], av = Ap[i];
+            typename Vec2<R>::T rv;
+            rv.x = R(-1) * av.x + bv.x;
+            rv.y = R(-1) * av.y + bv.y;
             r[i] = rv;
             p[i] = rv;
-            cx<double> rc = mk<double>(rv.x, rv.y);
+            cx<R> rc = mk<R>(rv.x, rv.y);
             acc = cadd(acc, cmul(cconj(rc), rc));
         }
-        re = acc.re;
+        re = double(acc.re);
     }
     store_partials(re, 0.0, false, lb, partials);
 }
 
 // x = alpha*p + x; r = (-alpha)*Ap + r; rr_new = |r|^2  (cg.jl:116-120)
-template <int NCOMP>
-__global__ void __launch_bounds__(256) k_cg_update_xr(double2* __restrict__ x, double2* __restrict__ r,
-                                                      const double2* __restrict__ p, const double2* __restrict__ Ap,
-                                                      int V, double* partials, CGState* st) {
+template <int NCOMP, typename R = double>
+__global__ void __launch_bounds__(256) k_cg_update_xr(typename Vec2<R>::T* __restrict__ x,
+                                                      typename Vec2<R>::T* __restrict__ r,
+                                                      const typename Vec2<R>::T* __restrict__ p,
+                                                      const typename Vec2<R>::T* __restrict__ Ap, int V,
+                                                      double* partials, CGState* st) {
     if (st->done) return;
-    const double alpha = st->alpha;
-    const double malpha = -alpha;
+    const R alpha = R(st->alpha);
+    const R malpha = -alpha;
     const int lb = blockIdx.x;
     const int s = lb * 256 + threadIdx.x;
     double re = 0.0;
     if (s < V) {
-        cx<double> acc = mk<double>(0.0, 0.0);
+        cx<R> acc = mk<R>(R(0), R(0));
 #pragma unroll
         for (int k = 0; k < NCOMP; ++k) {
             const long i = (long)k * V + s;
-            double2 pv = p[i], xv = x[i], av = Ap[i], rv = r[i];
-            xv = make_double2(alpha * pv.x + 1.0 * xv.x, alpha * pv.y + 1.0 * xv.y);
-            rv = make_double2(malpha * av.x + 1.0 * rv.x, malpha * av.y + 1.0 * rv.y);
+            auto pv = p[i], xv = x[i], av = Ap[i], rv = r[i];
+            xv.x = alpha * pv.x + R(1) * xv.x;
+            xv.y = alpha * pv.y + R(1) * xv.y;
+            rv.x = malpha * av.x + R(1) * rv.x;
+            rv.y = malpha * av.y + R(1) * rv.y;
             x[i] = xv;
             r[i] = rv;
-            cx<double> rc = mk<double>(rv.x, rv.y);
+            cx<R> rc = mk<R>(rv.x, rv.y);
             acc = cadd(acc, cmul(cconj(rc), rc));
         }
-        re = acc.re;
+        re = double(acc.re);
     }
     store_partials(re, 0.0, false, lb, partials);
 }
 
 // p = 1*r + beta*p  (cg.jl:130-131)
-template <int NCOMP>
-__global__ void __launch_bounds__(256) k_cg_update_p(const double2* __restrict__ r, double2* __restrict__ p, int V,
-                                                     CGState* st) {
+template <int NCOMP, typename R = double>
+__global__ void __launch_bounds__(256) k_cg_update_p(const typename Vec2<R>::T* __restrict__ r,
+                                                     typename Vec2<R>::T* __restrict__ p, int V, CGState* st) {
     if (st->done) return;
-    const double beta = st->beta;
+    const R beta = R(st->beta);
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s < V) {
 #pragma unroll
         for (int k = 0; k < NCOMP; ++k) {
             const long i = (long)k * V + s;
-            double2 rv = r[i], pv = p[i];
-            p[i] = make_double2(1.0 * rv.x + beta * pv.x, 1.0 * rv.y + beta * pv.y);
+            auto rv = r[i], pv = p[i];
+            pv.x = R(1) * rv.x + beta * pv.x;
+            pv.y = R(1) * rv.y + beta * pv.y;
+            p[i] = pv;
         }
     }
 }
@@ -317,6 +344,37 @@ int launch_cg_update_p(lmx_context* ctx, const double2* r, double2* p, CGState* 
         hipLaunchKernelGGL(k_cg_update_p<12>, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, r, p, (int)ctx->V, st);
     else
         hipLaunchKernelGGL(k_cg_update_p<8>, dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, r, p, (int)ctx->V, st);
+    LMX_HIP(hipGetLastError());
+    return LMX_OK;
+}
+
+// The same CG vector kernels on ComplexF32 fields (NC = 3).
+int launch_dot_f32(lmx_context* ctx, const float2* A, const float2* B, int fin_kind, CGState* st, double* out2) {
+    const int nb = nblocks(ctx->V);
+    const bool want_im = fin_kind == FIN_OUT;
+    hipLaunchKernelGGL((k_dot<12, float>), dim3(nb), dim3(256), 0, ctx->stream, A, B, (int)ctx->V, ctx->partials,
+                       want_im ? 1 : 0, st);
+    LMX_HIP(hipGetLastError());
+    return launch_finalize(ctx, nb, fin_kind, want_im, st, out2);
+}
+
+int launch_cg_setup_r_f32(lmx_context* ctx, const float2* b, const float2* Ap, float2* r, float2* p, CGState* st) {
+    hipLaunchKernelGGL((k_cg_setup_r<12, float>), dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, b, Ap, r, p,
+                       (int)ctx->V, ctx->partials, st);
+    LMX_HIP(hipGetLastError());
+    return launch_finalize(ctx, nblocks(ctx->V), FIN_RR0, false, st, nullptr);
+}
+
+int launch_cg_update_xr_f32(lmx_context* ctx, float2* x, float2* r, const float2* p, const float2* Ap, CGState* st) {
+    hipLaunchKernelGGL((k_cg_update_xr<12, float>), dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, x, r, p, Ap,
+                       (int)ctx->V, ctx->partials, st);
+    LMX_HIP(hipGetLastError());
+    return launch_finalize(ctx, nblocks(ctx->V), FIN_RRNEW, false, st, nullptr);
+}
+
+int launch_cg_update_p_f32(lmx_context* ctx, const float2* r, float2* p, CGState* st) {
+    hipLaunchKernelGGL((k_cg_update_p<12, float>), dim3(nblocks(ctx->V)), dim3(256), 0, ctx->stream, r, p,
+                       (int)ctx->V, st);
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }

@@ -21,7 +21,9 @@ __device__ __forceinline__ long padded_site(const int x[4], const Pad& p) {
     return (long)(x[0] + p.nw) + (long)p.P[0] * ((x[1] + p.nw) + (long)p.P[1] * ((x[2] + p.nw) + (long)p.P[2] * (x[3] + p.nw)));
 }
 
-__global__ void __launch_bounds__(256) k_spinor_import(double2* __restrict__ dst, const double2* __restrict__ src,
+// T2: double2 (ComplexF64 fields) or float2 (ComplexF32), R its real type.
+template <typename T2>
+__global__ void __launch_bounds__(256) k_spinor_import(T2* __restrict__ dst, const T2* __restrict__ src,
                                                        Geom g, Pad p, int nrhs, int ncomp) {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= (long)g.V * nrhs) return;
@@ -29,12 +31,13 @@ __global__ void __launch_bounds__(256) k_spinor_import(double2* __restrict__ dst
     const int r = (int)(t / g.V);
     int x[4];
     site_coords(s, g, x);
-    const double2* a = src + (long)r * p.Vp * ncomp + padded_site(x, p) * ncomp;
-    double2* o = dst + (long)r * ncomp * g.V;
+    const T2* a = src + (long)r * p.Vp * ncomp + padded_site(x, p) * ncomp;
+    T2* o = dst + (long)r * ncomp * g.V;
     for (int k = 0; k < ncomp; ++k) o[(long)k * g.V + s] = a[k];
 }
 
-__global__ void __launch_bounds__(256) k_spinor_export(double2* __restrict__ dst, const double2* __restrict__ src,
+template <typename T2, typename R>
+__global__ void __launch_bounds__(256) k_spinor_export(T2* __restrict__ dst, const T2* __restrict__ src,
                                                        Geom g, Pad p, int nrhs, int ncomp) {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= p.Vp * nrhs) return;
@@ -60,17 +63,20 @@ __global__ void __launch_bounds__(256) k_spinor_export(double2* __restrict__ dst
         x[d] = c;
     }
     const int s = x[0] + g.stride[1] * x[1] + g.stride[2] * x[2] + g.stride[3] * x[3];
-    const double2* in = src + (long)r * ncomp * g.V;
-    double2* o = dst + (long)r * p.Vp * ncomp + ps * ncomp;
+    const T2* in = src + (long)r * ncomp * g.V;
+    T2* o = dst + (long)r * p.Vp * ncomp + ps * ncomp;
     for (int k = 0; k < ncomp; ++k) {
-        double2 v = in[(long)k * g.V + s];
-        cx<double> z = mk<double>(v.x, v.y);
+        const T2 v = in[(long)k * g.V + s];
+        cx<R> z = mk<R>(v.x, v.y);
         for (int d = 0; d < 4; ++d) {
             if (w[d] == 0 || g.phase_one[d]) continue;
-            cx<double> ph = w[d] > 0 ? mk<double>(g.ph_re[d], g.ph_im[d]) : mk<double>(g.iph_re[d], g.iph_im[d]);
+            cx<R> ph = w[d] > 0 ? mk<R>(R(g.ph_re[d]), R(g.ph_im[d])) : mk<R>(R(g.iph_re[d]), R(g.iph_im[d]));
             z = cmul(z, ph);
         }
-        o[k] = make_double2(z.re, z.im);
+        T2 out;
+        out.x = z.re;
+        out.y = z.im;
+        o[k] = out;
     }
 }
 
@@ -134,26 +140,42 @@ static int check_nw(const lmx_context* ctx, int nw) {
     return LMX_OK;
 }
 
-int spinor_import(lmx_context* ctx, void* dst, const void* src, int nw, int nrhs) {
+template <typename T2>
+static int spinor_import_t(lmx_context* ctx, void* dst, const void* src, int nw, int nrhs) {
     LMX_TRY(check_nw(ctx, nw));
     Pad p = make_pad(ctx, nw);
     const long n = ctx->V * (long)nrhs;
     if (n == 0) return LMX_OK;
-    hipLaunchKernelGGL(k_spinor_import, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (double2*)dst,
-                       (const double2*)src, ctx->g, p, nrhs, spinor_ncomp(ctx));
+    hipLaunchKernelGGL((k_spinor_import<T2>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (T2*)dst,
+                       (const T2*)src, ctx->g, p, nrhs, spinor_ncomp(ctx));
     LMX_HIP(hipGetLastError());
     return LMX_OK;
 }
 
-int spinor_export(lmx_context* ctx, void* dst, int nw, const void* src, int nrhs) {
+template <typename T2, typename R>
+static int spinor_export_t(lmx_context* ctx, void* dst, int nw, const void* src, int nrhs) {
     LMX_TRY(check_nw(ctx, nw));
     Pad p = make_pad(ctx, nw);
     const long n = p.Vp * nrhs;
     if (n == 0) return LMX_OK;
-    hipLaunchKernelGGL(k_spinor_export, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (double2*)dst,
-                       (const double2*)src, ctx->g, p, nrhs, spinor_ncomp(ctx));
+    hipLaunchKernelGGL((k_spinor_export<T2, R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (T2*)dst, (const T2*)src, ctx->g, p, nrhs, spinor_ncomp(ctx));
     LMX_HIP(hipGetLastError());
     return LMX_OK;
+}
+
+int spinor_import(lmx_context* ctx, void* dst, const void* src, int nw, int nrhs) {
+    return spinor_import_t<double2>(ctx, dst, src, nw, nrhs);
+}
+int spinor_export(lmx_context* ctx, void* dst, int nw, const void* src, int nrhs) {
+    return spinor_export_t<double2, double>(ctx, dst, nw, src, nrhs);
+}
+// ComplexF32 fields: LatticeMatrix{ComplexF32}.A <-> float2 [12][V]
+int spinor_import_f32(lmx_context* ctx, void* dst, const void* src, int nw, int nrhs) {
+    return spinor_import_t<float2>(ctx, dst, src, nw, nrhs);
+}
+int spinor_export_f32(lmx_context* ctx, void* dst, int nw, const void* src, int nrhs) {
+    return spinor_export_t<float2, float>(ctx, dst, nw, src, nrhs);
 }
 
 int links_import(lmx_context* ctx, void* dst, const void* const src[4], int nw) {

@@ -185,6 +185,8 @@ struct LoArgs {
     const float* pprev_norm;
     void* pnext;
     float* pnext_norm;
+    // a device CG state whose done flag ends the launch early (the FP32 cg!)
+    const CGState* st;
 };
 int launch_wilson_lo(lmx_context* ctx, const LoArgs& a);
 
@@ -216,6 +218,11 @@ int launch_cg_setup_r(lmx_context* ctx, const double2* b, const double2* Ap, dou
 int launch_cg_update_xr(lmx_context* ctx, double2* x, double2* r, const double2* p, const double2* Ap, CGState* st);
 int launch_cg_update_p(lmx_context* ctx, const double2* r, double2* p, CGState* st);
 int launch_cg_init_state(lmx_context* ctx, CGState* st, double rtol, double atol, int maxiter);
+// the same on ComplexF32 fields (float2 [12][V]; NC = 3)
+int launch_dot_f32(lmx_context* ctx, const float2* A, const float2* B, int fin_kind, CGState* st, double* out2);
+int launch_cg_setup_r_f32(lmx_context* ctx, const float2* b, const float2* Ap, float2* r, float2* p, CGState* st);
+int launch_cg_update_xr_f32(lmx_context* ctx, float2* x, float2* r, const float2* p, const float2* Ap, CGState* st);
+int launch_cg_update_p_f32(lmx_context* ctx, const float2* r, float2* p, CGState* st);
 
 // ---- pullback / TA (force.hip) ----
 int launch_pullback_f64(lmx_context* ctx, double2* dU, const double2* dres, const double2* psi, double coefficient);

@@ -643,6 +643,7 @@ template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int FPM,
 __global__ void __launch_bounds__(256) k_lds(Out out, In in, Lk U, Cl F, Aux aux, Ex ex, Geom g, double kappa_d,
                                              double ccoef_d, int nrhs, int TL, int ZS, EpiArgs epi) {
     using R = float;
+    if (epi.st != nullptr && epi.st->done) return;
     if (epi.gate != nullptr && *epi.gate == 0) return;
     __shared__ typename In::Slot slot[2];
     __shared__ double wpart[EPI != EPI_NONE ? kMaxTL : 1][4];
@@ -1296,7 +1297,8 @@ int launch_lo_nr(lmx_context* ctx, const LoArgs& a, Lk U, Cl F, bool clover) {
         const int ngroups = (n + NR - 1) / NR;
         Out o = Out::at(a.out, a.out_norm, V, r0);
         In i = In::at(a.in, a.in_norm, V, r0);
-        EpiArgs epi{a.partials ? a.partials + (long)r0 * a.pstride : nullptr, a.pstride, nullptr, a.gate};
+        EpiArgs epi{a.partials ? a.partials + (long)r0 * a.pstride : nullptr, a.pstride, const_cast<CGState*>(a.st),
+                    a.gate};
         const NoEx none{};
         if (a.epi == EPI_XR) {
             In p = In::at(a.aux, a.aux_norm, V, r0);

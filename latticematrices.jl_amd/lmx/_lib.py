@@ -98,7 +98,10 @@ SIGNATURES = {
     "lmx_cg_params_default": (None, [ctypes.POINTER(CGParams)]),
     "lmx_cg_ddagd": (I, [P, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
     "lmx_cg_ddagd_noalloc": (I, [P, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
+    "lmx_cg_ddagd_f32": (I, [P, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
     "lmx_cg_ddagd_padded": (I, [P, P, P, I, P, ctypes.c_size_t, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
+    "lmx_cg_ddagd_padded_f32": (I, [P, P, P, I, P, ctypes.c_size_t, ctypes.POINTER(CGParams),
+                                    ctypes.POINTER(CGResultC)]),
     "lmx_cg_generic": (I, [P, APPLY_FN, P, P, P, P, P, P, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),
     "lmx_cg_batched_workspace": (I, [P, I, ctypes.POINTER(CGParams), ctypes.POINTER(ctypes.c_size_t)]),
     "lmx_cg_batched": (I, [P, P, P, I, P, ctypes.c_size_t, ctypes.POINTER(CGParams), ctypes.POINTER(CGResultC)]),

@@ -680,13 +680,47 @@ def _aliases(fields):
                 raise ArgumentError(_lib.LMX_ERR_ALIAS, f"CG fields {names[i]} and {names[j]} must not alias")
 
 
+def _cg_f32(x, A, b, r, p, Ap, params) -> CGResult:
+    """cg! on single-precision fields (float32 [1, 12, V, 2], spinor_f32):
+    A must be DdagDOp of a Wilson(-clover) operator; its temp is used when it
+    is such a field, else one is allocated."""
+    names = ("x", "b", "r", "p", "Ap")
+    fields = (x, b, r, p, Ap)
+    if not isinstance(A, DdagDOp) or not isinstance(A.D, WilsonDiracOperator4D):
+        raise ArgumentError(_lib.LMX_ERR_UNSUPPORTED, "cg_ on single-precision fields is built for DdagDOp of a "
+                                                      "Wilson(-clover) operator")
+    lat = A.D.lattice
+    for n, t in zip(names, fields):
+        if not (isinstance(t, torch.Tensor) and t.dtype == torch.float32 and t.is_contiguous()
+                and tuple(t.shape) == (1, 12, lat.V, 2) and t.device == lat.device):
+            raise ArgumentError(_lib.LMX_ERR_ARG, f"CG field {n} must be a contiguous float32 [1, 12, V, 2] field "
+                                                  "on the operator's device")
+    for i in range(5):
+        for j in range(i + 1, 5):
+            if fields[i].data_ptr() == fields[j].data_ptr():
+                raise ArgumentError(_lib.LMX_ERR_ALIAS, f"CG fields {names[i]} and {names[j]} must not alias")
+    tmp = A.temp if (isinstance(A.temp, torch.Tensor) and A.temp.dtype == torch.float32
+                     and tuple(A.temp.shape) == (1, 12, lat.V, 2)) else spinor_f32(lat, 1)
+    _refresh_low_precision(A.D)
+    res = _lib.CGResultC()
+    check(lib().lmx_cg_ddagd_f32(A.D._op, ctypes.c_void_p(_ptr(x)), ctypes.c_void_p(_ptr(b)),
+                                 ctypes.c_void_p(_ptr(r)), ctypes.c_void_p(_ptr(p)), ctypes.c_void_p(_ptr(Ap)),
+                                 ctypes.c_void_p(_ptr(tmp)), ctypes.byref(params), ctypes.byref(res)))
+    return CGResult.from_c(res)
+
+
 def cg_(x, A, b, r, p, Ap, rtol=1e-10, atol=0.0, maxiter=5000, check_every=0) -> CGResult:
-    """cg!(x, A, b, r, p, Ap; rtol, atol, maxiter) (src/LinearAlgebras/cg.jl:73-137)."""
-    _aliases((x, b, r, p, Ap))
-    params = _lib.default_params(rtol=float(rtol), atol=float(atol), maxiter=int(maxiter),
-                                 check_every=int(check_every))
+    """cg!(x, A, b, r, p, Ap; rtol, atol, maxiter) (src/LinearAlgebras/cg.jl:73-137).
+    The element type follows the fields: SpinorField (ComplexF64), or
+    single-precision fields (spinor_f32 tensors, ComplexF32) for D^dagger D of
+    a Wilson(-clover) operator."""
     if int(maxiter) < 0:
         raise ArgumentError(_lib.LMX_ERR_ARG, "maxiter must be nonnegative")
+    params = _lib.default_params(rtol=float(rtol), atol=float(atol), maxiter=int(maxiter),
+                                 check_every=int(check_every))
+    if isinstance(x, torch.Tensor):
+        return _cg_f32(x, A, b, r, p, Ap, params)
+    _aliases((x, b, r, p, Ap))
     res = _lib.CGResultC()
     if isinstance(A, DdagDOp) and isinstance(A.D, WilsonDiracOperator4D) and A.temp is not None:
         check(lib().lmx_cg_ddagd(A.D._op, x.ptr, b.ptr, r.ptr, p.ptr, Ap.ptr, A.temp.ptr, ctypes.byref(params),

@@ -8,7 +8,8 @@ the oracle:
   mul!(C, D, psi)      lmx_wilson_apply_padded (D, D', Donly, Donly', clover)
   cg!(x, DdagDOp(D, temp), b, r, p, Ap)
                        lmx_cg_ddagd_padded (x, b padded; the six work fields in
-                       the operator's workspace); x's halo is then set_halo!'s
+                       the operator's workspace); x's halo is then set_halo!'s;
+                       on LatticeMatrix{ComplexF32}: lmx_cg_ddagd_padded_f32
   link write           lmx_links_import + lmx_wilson_update_links
   reverse rule of mul! (dU += pullback)
                        lmx_links_import (dU), lmx_spinor_import (dresult, psi),
@@ -96,6 +97,15 @@ class Binding:
         _lib.check(lib().lmx_cg_ddagd_padded(self.op, P(x.data_ptr()), P(b.data_ptr()), self.nw,
                                              P(self.work.data_ptr()), self.work.numel(), ctypes.byref(params),
                                              ctypes.byref(res)))
+        return res
+
+    def cg_f32(self, x, b, rtol=1e-5, atol=0.0, maxiter=5000):
+        """cg! on LatticeMatrix{ComplexF32} (complex64 arrays), the same workspace."""
+        params = _lib.default_params(rtol=rtol, atol=atol, maxiter=maxiter)
+        res = _lib.CGResultC()
+        _lib.check(lib().lmx_cg_ddagd_padded_f32(self.op, P(x.data_ptr()), P(b.data_ptr()), self.nw,
+                                                 P(self.work.data_ptr()), self.work.numel(), ctypes.byref(params),
+                                                 ctypes.byref(res)))
         return res
 
     def slot(self, i):
@@ -225,5 +235,36 @@ def test_replay_link_write_pullback_and_dsfdu(gpu):
         print(f"dSFdU: CG iterations {res.iterations} (oracle {ro.iterations})", flush=True)
         assert res.converged and res.iterations == ro.iterations
         assert np.array_equal(links_from_padded(dfdU, L, nw), Go)
+    finally:
+        B.close()
+
+
+@pytest.mark.parametrize("csw", [0.0, 1.0])
+def test_replay_cg_f32(gpu, csw):
+    # cg! on LatticeMatrix{ComplexF32} (complex64 padded arrays): FP32 iterates,
+    # so the bar is the FP64 true residual of the solution, the iteration
+    # count near the FP64 oracle's at the same rtol, and x's halo as set_halo!
+    nw, L, kappa, rtol = 1, (4, 4, 4, 8), 0.12, 1e-5
+    V = int(np.prod(L))
+    U = gen.links(L, "haar", 715)
+    rng = np.random.default_rng(716)
+    b = (rng.standard_normal((V, 4, 3)) + 1j * rng.standard_normal((V, 4, 3))).astype(np.complex64)
+    B = Binding(L, 3, padded_links(U, L, nw, gpu), nw, kappa, csw, gpu)
+    try:
+        b_d = padded_spinor(b, L, nw, gpu)
+        assert b_d.dtype == torch.complex64
+        x = torch.zeros_like(b_d)
+        res = B.cg_f32(x, b_d, rtol=rtol)
+        Fo = orc.clover_field_strength(U, L) if csw else None
+        op = orc.NormalOp(orc.OP_DDAGD_CLOVER if csw else orc.OP_DDAGD_WILSON, L, U=U, F=Fo, kappa=kappa, csw=csw,
+                          phases=PHASES)
+        b64 = b.astype(np.complex128)
+        xo, ro = orc.cg(op, b64, rtol=rtol)
+        xs = core(x, L, nw).reshape(V, 4, 3)
+        true_rel = np.linalg.norm(op.apply(xs.astype(np.complex128)) - b64) / np.linalg.norm(b64)
+        print(f"  csw={csw}: ComplexF32 cg! iterations {res.iterations} (FP64 oracle {ro.iterations}), FP64 true rel "
+              f"{true_rel:.3e}", flush=True)
+        assert res.converged and abs(res.iterations - ro.iterations) <= 3 and true_rel <= 3 * rtol
+        assert np.array_equal(x.cpu().numpy(), dr.halo_fill(xs, L, nw, PHASES).astype(np.complex64))
     finally:
         B.close()
