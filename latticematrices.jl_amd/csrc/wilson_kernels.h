@@ -188,14 +188,12 @@ enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
 #endif
 
 // Fused epilogues: EPI_PAP   Re <aux, out> (the CG's p.Ap on D^dagger, FP64 CG);
-//                  EPI_NORM  |out|^2 (p.Ap = |D p|^2 on D; not instantiated
-//                            for the low-precision launches, which use EPI_TP);
 //                  EPI_XR    out = Ap is not stored: r -= alpha Ap, x += alpha p
 //                            (p = aux) and |r|^2 (mixed solver, on D^dagger);
 //                  EPI_TP    the stencil input is r: out = t = D r + beta t in
 //                            place (= D p by linearity), p = r + beta p_prev
 //                            written at the site, and |t|^2 (mixed solver, on D).
-enum { EPI_NONE = 0, EPI_PAP = 1, EPI_NORM = 2, EPI_XR = 3, EPI_TP = 4 };
+enum { EPI_NONE = 0, EPI_PAP = 1, EPI_XR = 3, EPI_TP = 4 };
 
 struct NoEx {};
 // Extra fields of EPI_XR, all offset to the launch's first RHS.  The x
@@ -350,7 +348,7 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
             red = red + t.re;
         }
     }
-    if constexpr (EPI == EPI_NORM || EPI == EPI_TP) {
+    if constexpr (EPI == EPI_TP) {
         R site = R(0);  // per-site sum in working precision, accumulated over sites in FP64
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
@@ -1166,7 +1164,7 @@ inline Shape pick_shape(const lmx_context* ctx, int ngroups, int min_blocks, int
 // instantiates the per-site kernel for +-1 phases too (else the GP kernel
 // covers lattices whose 3-D volume is not a multiple of 256).  The fused
 // epilogues exist where D^dagger D uses them: EPI_PAP and EPI_XR on
-// D^dagger, EPI_NORM and EPI_TP on D.
+// D^dagger, EPI_TP on D.
 template <class In, class Lk, class Out, class Cl, class Aux, class Ex, int NR, int EPI, bool SITE_NG, bool DONLY_OK>
 int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex, double kappa, double ccoef,
              int ngroups, int nrhs, int flags, bool clover, const EpiArgs& epi, int min_blocks, int tl_hint = 0,
@@ -1182,7 +1180,7 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
     hipStream_t st = ctx->stream;
     // FP64 streaming Wilson / hopping-only launches use the two-lanes-per-site kernel
     constexpr bool kTwoLane = LMX_TWO_LANE && std::is_same<In, SpF64In>::value && NR == 1;
-    constexpr bool kOnD = EPI == EPI_NORM || EPI == EPI_TP;
+    constexpr bool kOnD = EPI == EPI_TP;
     if (EPI != EPI_NONE && (donly || dag == kOnD)) {
         set_error("fused reduction %d not built for this operator (p.Ap and x/r updates on D^dagger, |D p|^2 on D)",
                   EPI);
