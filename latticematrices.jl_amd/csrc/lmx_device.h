@@ -180,46 +180,59 @@ __device__ __forceinline__ T ldv(const T* ubase, int s) {
     return *(const T*)((const char*)ubase + (unsigned)s * (unsigned)sizeof(T));
 }
 
-// Streamed loads (nontemporal): for fields a launch reads once at the site
-// itself (the fused epilogues' r, t, p, x_inner), so that they do not evict
-// the neighbour spinors and links the stencil re-reads from L2.
+// The low-precision fields are read through global address-space pointers:
+// the compiler then emits global loads, not flat ones (a flat load also
+// counts against the LDS counter, so it serialises with the LDS-staged
+// kernels' shared-memory traffic), and forms each address as a 32-bit lane
+// offset from the uniform base.  The base must be uniform, as for ldv.
+template <typename T>
+using gptr_t = __attribute__((address_space(1))) const T*;
+// element s of the field at base, read as the builtin type VT (a global
+// address-space lvalue of a HIP vector type cannot be copied directly)
+template <typename T, typename VT, bool NT>
+__device__ __forceinline__ VT ldg_raw(const T* base, int s) {
+    const gptr_t<T> g = (gptr_t<T>)base;
+    using GV = __attribute__((address_space(1))) const VT*;
+    const GV a = (GV)((__attribute__((address_space(1))) const char*)g + (unsigned)s * (unsigned)sizeof(T));
+    if constexpr (NT) return __builtin_nontemporal_load(a);
+    return *a;
+}
+__device__ __forceinline__ int ldg(const int* base, int s) { return ldg_raw<int, int, false>(base, s); }
+__device__ __forceinline__ float ldg(const float* base, int s) { return ldg_raw<float, float, false>(base, s); }
+__device__ __forceinline__ int4 ldg(const int4* base, int s) {
+    const lmx_iv4 v = ldg_raw<int4, lmx_iv4, false>(base, s);
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 ldg(const float2* base, int s) {
+    const lmx_fv2 v = ldg_raw<float2, lmx_fv2, false>(base, s);
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 ldg(const float4* base, int s) {
+    const lmx_fv4 v = ldg_raw<float4, lmx_fv4, false>(base, s);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Streamed loads (ldg with the nontemporal hint): for fields a launch reads
+// once at the site itself (the fused epilogues' r, t, p, x_inner), so that
+// they do not evict the neighbour spinors and links the stencil re-reads from
+// L2.
 #ifndef LMX_NT_STREAM
 #define LMX_NT_STREAM 1
 #endif
-__device__ __forceinline__ int4 ldv_stream(const int4* ubase, int s) {
-    const int4* a = (const int4*)((const char*)ubase + (unsigned)s * 16u);
-#if LMX_NT_STREAM
-    const lmx_iv4 v = __builtin_nontemporal_load((const lmx_iv4*)a);
+__device__ __forceinline__ int4 ldg_stream(const int4* base, int s) {
+    const lmx_iv4 v = ldg_raw<int4, lmx_iv4, LMX_NT_STREAM != 0>(base, s);
     return make_int4(v.x, v.y, v.z, v.w);
-#else
-    return *a;
-#endif
 }
-__device__ __forceinline__ float ldv_stream(const float* ubase, int s) {
-    const float* a = (const float*)((const char*)ubase + (unsigned)s * 4u);
-#if LMX_NT_STREAM
-    return __builtin_nontemporal_load(a);
-#else
-    return *a;
-#endif
+__device__ __forceinline__ float ldg_stream(const float* base, int s) {
+    return ldg_raw<float, float, LMX_NT_STREAM != 0>(base, s);
 }
-__device__ __forceinline__ float2 ldv_stream(const float2* ubase, int s) {
-    const float2* a = (const float2*)((const char*)ubase + (unsigned)s * 8u);
-#if LMX_NT_STREAM
-    const lmx_fv2 v = __builtin_nontemporal_load((const lmx_fv2*)a);
+__device__ __forceinline__ float2 ldg_stream(const float2* base, int s) {
+    const lmx_fv2 v = ldg_raw<float2, lmx_fv2, LMX_NT_STREAM != 0>(base, s);
     return make_float2(v.x, v.y);
-#else
-    return *a;
-#endif
 }
-__device__ __forceinline__ float4 ldv_stream(const float4* ubase, int s) {
-    const float4* a = (const float4*)((const char*)ubase + (unsigned)s * 16u);
-#if LMX_NT_STREAM
-    const lmx_fv4 v = __builtin_nontemporal_load((const lmx_fv4*)a);
+__device__ __forceinline__ float4 ldg_stream(const float4* base, int s) {
+    const lmx_fv4 v = ldg_raw<float4, lmx_fv4, LMX_NT_STREAM != 0>(base, s);
     return make_float4(v.x, v.y, v.z, v.w);
-#else
-    return *a;
-#endif
 }
 
 struct SpF64In {
@@ -277,7 +290,7 @@ struct SpF32In {
     __device__ __forceinline__ void stage(int r, int s, Slot& sl, int lane) const {
         const float2* q = p + (size_t)r * 12 * V;
 #pragma unroll
-        for (int k = 0; k < 12; ++k) sl.w[k][lane] = ldv(q + (size_t)k * V, s);
+        for (int k = 0; k < 12; ++k) sl.w[k][lane] = ldg(q + (size_t)k * V, s);
     }
     __device__ static __forceinline__ void lds_ld12s(const Slot& sl, int lane, cx<float>* z, float& c) {
 #pragma unroll
@@ -291,7 +304,7 @@ struct SpF32In {
         const float2* q = p + (size_t)r * 12 * V;
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            float2 v = ldv(q + (size_t)k * V, s);
+            float2 v = ldg(q + (size_t)k * V, s);
             z[k] = mk<float>(v.x, v.y);
         }
         c = 1.0f;
@@ -305,7 +318,7 @@ struct SpF32In {
         const float2* q = p + (size_t)r * 12 * V;
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            float2 v = ldv_stream(q + (size_t)k * V, s);
+            float2 v = ldg_stream(q + (size_t)k * V, s);
             z[k] = mk<float>(v.x, v.y);
         }
         c = 1.0f;
@@ -341,7 +354,7 @@ struct SpX4In {
         const float4* q = p + (size_t)r * 6 * V;
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-            float4 v = ldv(q + (size_t)m * V, s);
+            float4 v = ldg(q + (size_t)m * V, s);
             z[2 * m] = mk<float>(v.x, v.y);
             z[2 * m + 1] = mk<float>(v.z, v.w);
         }
@@ -350,7 +363,7 @@ struct SpX4In {
         const float4* q = p + (size_t)r * 6 * V;
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-            float4 v = ldv_stream(q + (size_t)m * V, s);
+            float4 v = ldg_stream(q + (size_t)m * V, s);
             z[2 * m] = mk<float>(v.x, v.y);
             z[2 * m + 1] = mk<float>(v.z, v.w);
         }
@@ -397,8 +410,8 @@ struct SpH16In {
     __device__ __forceinline__ void stage(int r, int s, Slot& sl, int lane) const {
         const int4* q = p + (size_t)r * 3 * V;
 #pragma unroll
-        for (int g = 0; g < 3; ++g) sl.w[g][lane] = ldv(q + (size_t)g * V, s);
-        sl.c[lane] = ldv(scale + (size_t)r * V, s);
+        for (int g = 0; g < 3; ++g) sl.w[g][lane] = ldg(q + (size_t)g * V, s);
+        sl.c[lane] = ldg(scale + (size_t)r * V, s);
     }
     // integer components (as floats) and the site scale, as ld12s
     __device__ static __forceinline__ void lds_ld12s(const Slot& sl, int lane, cx<float>* z, float& c) {
@@ -415,10 +428,10 @@ struct SpH16In {
     // integer components (as floats) and the site scale: value = z * c
     __device__ __forceinline__ void ld12s(int r, int s, cx<float>* z, float& c) const {
         const int4* q = p + (size_t)r * 3 * V;
-        c = ldv(scale + (size_t)r * V, s);
+        c = ldg(scale + (size_t)r * V, s);
 #pragma unroll
         for (int g = 0; g < 3; ++g) {
-            int4 v = ldv(q + (size_t)g * V, s);
+            int4 v = ldg(q + (size_t)g * V, s);
             z[4 * g + 0] = mk<float>(h16_lo(v.x), h16_hi(v.x));
             z[4 * g + 1] = mk<float>(h16_lo(v.y), h16_hi(v.y));
             z[4 * g + 2] = mk<float>(h16_lo(v.z), h16_hi(v.z));
@@ -434,10 +447,10 @@ struct SpH16In {
     // streamed forms (fields read once, at the site itself)
     __device__ __forceinline__ void ld12s_stream(int r, int s, cx<float>* z, float& c) const {
         const int4* q = p + (size_t)r * 3 * V;
-        c = ldv_stream(scale + (size_t)r * V, s);
+        c = ldg_stream(scale + (size_t)r * V, s);
 #pragma unroll
         for (int g = 0; g < 3; ++g) {
-            int4 v = ldv_stream(q + (size_t)g * V, s);
+            int4 v = ldg_stream(q + (size_t)g * V, s);
             z[4 * g + 0] = mk<float>(h16_lo(v.x), h16_hi(v.x));
             z[4 * g + 1] = mk<float>(h16_lo(v.y), h16_hi(v.y));
             z[4 * g + 2] = mk<float>(h16_lo(v.z), h16_hi(v.z));
@@ -523,7 +536,7 @@ struct LinkF32 {
         const float2* q = p + (size_t)mu * 9 * V;
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            float2 v = ldv(q + (size_t)k * V, s);
+            float2 v = ldg(q + (size_t)k * V, s);
             u[k] = mk<R>(R(v.x), R(v.y));
         }
     }
@@ -537,9 +550,9 @@ struct LinkF16 {
     int V;
     template <typename R>
     __device__ __forceinline__ void ld9(int mu, int s, cx<R>* u) const {
-        const int4 a = ldv(p + (size_t)(2 * mu) * V, s);
-        const int4 b = ldv(p + (size_t)(2 * mu + 1) * V, s);
-        const int c = ldv((const int*)(p + (size_t)8 * V) + (size_t)mu * V, s);
+        const int4 a = ldg(p + (size_t)(2 * mu) * V, s);
+        const int4 b = ldg(p + (size_t)(2 * mu + 1) * V, s);
+        const int c = ldg((const int*)(p + (size_t)8 * V) + (size_t)mu * V, s);
         const int w[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c};
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
