@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace lmx {
 
 template <typename R>
@@ -118,7 +120,9 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // of V sites (components outermost, so a wave's load is one contiguous run):
 //   F64  double2[12][V]      F32  float2[12][V]
 //   H16  int4[3][V] + float[V]: components 4g..4g+3 as 16-bit fixed-point
-//        (re, im) pairs, value = q * scale[s], scale = max |component| / 32767
+//        (re, im) pairs, value = q * scale[s], scale = max |component| / 32752
+//        (32752, not 32767: a sum or difference of two components, the
+//        projected half-spinor, then stays within the FP16 range, 65504)
 // Indices are 32-bit within one right-hand side or link direction (12 V < 2^31).
 // ---------------------------------------------------------------------------
 // Output stores bypass the caches' normal allocation (nontemporal): the
@@ -383,11 +387,11 @@ struct SpX4Out {
     }
 };
 
-constexpr float kH16Max = 32767.0f;
+constexpr float kH16Max = 32752.0f;
 
 __device__ __forceinline__ float h16_lo(int w) { return float(int(unsigned(w) << 16) >> 16); }
 __device__ __forceinline__ float h16_hi(int w) { return float(w >> 16); }
-// round(re * inv), round(im * inv) (|.| <= 32767, nearest-even, one rounding
+// round(re * inv), round(im * inv) (|.| <= 32752, nearest-even, one rounding
 // of the exact product) as a 16-bit pair: x + 1.5 * 2^23 lies in [2^23, 2^24),
 // where the float spacing is 1, so the low 16 bits of its encoding are round(x)
 // in two's complement; one FMA per component and one byte permute per pair.
@@ -476,7 +480,7 @@ struct SpH16Out {
         float m = 0.0f;
 #pragma unroll
         for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(z[k].re), fabsf(z[k].im)));
-        // 1 ulp reciprocal: m * inv stays within 32767 * (1 + 2^-22) < 32767.5
+        // 1 ulp reciprocal: m * inv stays within 32752 * (1 + 2^-22) < 32752.5
         const float inv = m > 0.0f ? kH16Max * __builtin_amdgcn_rcpf(m) : 0.0f;
         int4* q = p + (size_t)r * 3 * V;
 #pragma unroll
@@ -559,6 +563,15 @@ struct LinkF16 {
             float2 v = h2f(w[k]);
             u[k] = mk<R>(R(v.x), R(v.y));
         }
+    }
+    // the nine entries as their (re, im) FP16 words
+    __device__ __forceinline__ void ld9w(int mu, int s, unsigned* w) const {
+        const int4 a = ldg(p + (size_t)(2 * mu) * V, s);
+        const int4 b = ldg(p + (size_t)(2 * mu + 1) * V, s);
+        const int c = ldg((const int*)(p + (size_t)8 * V) + (size_t)mu * V, s);
+        w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+        w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+        w[8] = c;
     }
 };
 
@@ -681,6 +694,136 @@ __device__ __forceinline__ void matvec_bwd(const cx<R> u[9], const cx<R> h[6], c
         r[2 * a + 1] = muladdmulti(w0, h[1], w1, h[3], w2, h[5]);
     }
 }
+
+// U h and U^dagger h from FP16 link entries, as FP16 pair products.
+// v_dot2_f32_f16 computes a.lo * b.lo + a.hi * b.hi + c from FP16 pairs (the
+// products are exact in FP32) into an FP32 accumulator, so one instruction
+// gives the real or imaginary part of a complex product with a link entry
+// w = (re, im) and sums it, where matvec_fwd with converted entries takes two
+// mixed FMAs.  Per half-spinor element h the operands are hp = (h.re, h.im)
+// and hs = (h.im, h.re); the minus sign of each sum is the VOP3P neg_hi
+// modifier.  The instruction is written out: the compiler's builtin selects
+// the modifier-free VOP2 form, which costs a negation per operand and a move
+// per accumulator.  A VALU read (or a write) of a dot result needs three wait
+// states after the dot, except as the next dot's accumulator: each block ends
+// with s_nop 2, so its outputs are safe to use as soon as it returns (the
+// compiler, which does not know the block, inserts nothing).  h is rounded to
+// FP16 (nearest even, relative error at most 2^-12, as for the links
+// themselves); the 16-bit stencils call these with h an integer sum of two
+// fixed-point components, |h| <= 2 * 32752 <= 65504, the FP16 maximum.
+typedef _Float16 lmx_hv2 __attribute__((ext_vector_type(2)));
+typedef float lmx_fv2x __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ lmx_hv2 pk16(float a, float b) { return __builtin_convertvector((lmx_fv2x){a, b}, lmx_hv2); }
+
+// one spin of U h (FWD: sum_j U[a][j] h_j) or U^dagger h (sum_j conj(U[j][a]) h_j)
+// for the three rows a: outputs (re, im) of rows 0, 1, 2; u the nine words
+// row-major; hp/hs the spin's three colours
+template <bool FWD>
+__device__ __forceinline__ void dot_spin(const unsigned u[9], const lmx_hv2 hp[3], const lmx_hv2 hs[3], cx<float> o[3]) {
+    float r0, i0, r1, i1, r2, i2;
+    if constexpr (FWD) {
+        asm(
+        "v_dot2_f32_f16 %0, %6, %15, 0 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %1, %6, %18, 0\n"
+        "v_dot2_f32_f16 %2, %9, %15, 0 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %3, %9, %18, 0\n"
+        "v_dot2_f32_f16 %4, %12, %15, 0 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %5, %12, %18, 0\n"
+        "v_dot2_f32_f16 %0, %7, %16, %0 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %1, %7, %19, %1\n"
+        "v_dot2_f32_f16 %2, %10, %16, %2 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %3, %10, %19, %3\n"
+        "v_dot2_f32_f16 %4, %13, %16, %4 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %5, %13, %19, %5\n"
+        "v_dot2_f32_f16 %0, %8, %17, %0 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %1, %8, %20, %1\n"
+        "v_dot2_f32_f16 %2, %11, %17, %2 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %3, %11, %20, %3\n"
+        "v_dot2_f32_f16 %4, %14, %17, %4 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %5, %14, %20, %5\n"
+        "s_nop 2\n"
+            : "=&v"(r0), "=&v"(i0), "=&v"(r1), "=&v"(i1), "=&v"(r2), "=&v"(i2)
+            : "v"(u[0]), "v"(u[1]), "v"(u[2]), "v"(u[3]), "v"(u[4]), "v"(u[5]), "v"(u[6]), "v"(u[7]), "v"(u[8]),
+              "v"(hp[0]), "v"(hp[1]), "v"(hp[2]), "v"(hs[0]), "v"(hs[1]), "v"(hs[2]));
+    } else {
+        asm(
+        "v_dot2_f32_f16 %0, %6, %15, 0\n"
+        "v_dot2_f32_f16 %1, %6, %18, 0 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %2, %7, %15, 0\n"
+        "v_dot2_f32_f16 %3, %7, %18, 0 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %4, %8, %15, 0\n"
+        "v_dot2_f32_f16 %5, %8, %18, 0 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %0, %9, %16, %0\n"
+        "v_dot2_f32_f16 %1, %9, %19, %1 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %2, %10, %16, %2\n"
+        "v_dot2_f32_f16 %3, %10, %19, %3 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %4, %11, %16, %4\n"
+        "v_dot2_f32_f16 %5, %11, %19, %5 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %0, %12, %17, %0\n"
+        "v_dot2_f32_f16 %1, %12, %20, %1 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %2, %13, %17, %2\n"
+        "v_dot2_f32_f16 %3, %13, %20, %3 neg_hi:[0,1,0]\n"
+        "v_dot2_f32_f16 %4, %14, %17, %4\n"
+        "v_dot2_f32_f16 %5, %14, %20, %5 neg_hi:[0,1,0]\n"
+        "s_nop 2\n"
+            : "=&v"(r0), "=&v"(i0), "=&v"(r1), "=&v"(i1), "=&v"(r2), "=&v"(i2)
+            : "v"(u[0]), "v"(u[1]), "v"(u[2]), "v"(u[3]), "v"(u[4]), "v"(u[5]), "v"(u[6]), "v"(u[7]), "v"(u[8]),
+              "v"(hp[0]), "v"(hp[1]), "v"(hp[2]), "v"(hs[0]), "v"(hs[1]), "v"(hs[2]));
+    }
+    o[0] = mk<float>(r0, i0);
+    o[1] = mk<float>(r1, i1);
+    o[2] = mk<float>(r2, i2);
+}
+// r[2a + sp] = (U h)[a] of spin sp (FWD) or (U^dagger h)[a]; matvec_fwd / matvec_bwd's sums
+template <bool FWD>
+__device__ __forceinline__ void matvec_h(const unsigned w[9], const cx<float> h[6], cx<float> r[6]) {
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp) {
+        lmx_hv2 hp[3], hs[3];
+        cx<float> o[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            hp[j] = pk16(h[2 * j + sp].re, h[2 * j + sp].im);
+            hs[j] = pk16(h[2 * j + sp].im, h[2 * j + sp].re);
+        }
+        dot_spin<FWD>(w, hp, hs, o);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r[2 * a + sp] = o[a];
+    }
+}
+
+// The link operand of one hop.  With 16-bit spinors and FP16 links (HP) it
+// keeps the nine FP16 words and multiplies with the pair products above;
+// otherwise it holds the entries converted to R.
+template <class Lk, typename R, bool HP>
+struct LinkOp {
+    cx<R> u[9];
+    __device__ __forceinline__ void load(const Lk& U, int mu, int s) { U.template ld9<R>(mu, s, u); }
+    template <class W>
+    __device__ __forceinline__ void from_words(const W& lw) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const float2 v = h2f(lw[k]);
+            u[k] = mk<R>(R(v.x), R(v.y));
+        }
+    }
+    __device__ __forceinline__ void fwd(const cx<R> h[6], cx<R> o[6]) const { matvec_fwd(u, h, o); }
+    __device__ __forceinline__ void bwd(const cx<R> h[6], cx<R> o[6]) const { matvec_bwd(u, h, o); }
+};
+template <class Lk>
+struct LinkOp<Lk, float, true> {
+    unsigned w[9];
+    __device__ __forceinline__ void load(const Lk& U, int mu, int s) { U.ld9w(mu, s, w); }
+    template <class W>
+    __device__ __forceinline__ void from_words(const W& lw) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = unsigned(lw[k]);
+    }
+    __device__ __forceinline__ void fwd(const cx<float> h[6], cx<float> o[6]) const { matvec_h<true>(w, h, o); }
+    __device__ __forceinline__ void bwd(const cx<float> h[6], cx<float> o[6]) const { matvec_h<false>(w, h, o); }
+};
+template <class In, class Lk>
+constexpr bool kHalfProducts = std::is_same<In, SpH16In>::value && std::is_same<Lk, LinkF16>::value;
 
 // Negate a half-spinor where `neg` holds (a boundary phase of -1).  Negation
 // commutes exactly with projection and the link product, so flipping the

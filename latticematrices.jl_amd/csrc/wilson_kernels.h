@@ -141,8 +141,10 @@ __device__ __forceinline__ void hop_half(cx<R> (&acc)[NR][12], const In& in, con
     bool w;
     const int t = FWD ? nbr_fwd(s, x, MU, g, w) : nbr_bwd(s, x, MU, g, w);
     const bool ng = w && g.phase_neg[MU];
-    cx<R> u[9];
-    U.template ld9<R>(MU, FWD ? s : t, u);
+    // general phases rotate the components, so the projected half-spinor can
+    // leave the FP16 range: those launches keep the converted link entries
+    LinkOp<Lk, R, kHalfProducts<In, Lk> && !GP> u;
+    u.load(U, MU, FWD ? s : t);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         cx<R> psi[12], h[6], o[6];
@@ -161,9 +163,9 @@ __device__ __forceinline__ void hop_half(cx<R> (&acc)[NR][12], const In& in, con
         // the general-phase kernel), so only t can wrap with a sign
         if (!GP && (MU == 3 || !kSpatialOne<R>)) neg_select(h, ng);
         if (FWD)
-            matvec_fwd(u, h, o);
+            u.fwd(h, o);
         else
-            matvec_bwd(u, h, o);
+            u.bwd(h, o);
         reconstruct_add_s<PM, MU>(acc[r], o, c);
     }
     pin_all(acc);
@@ -449,20 +451,28 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
     x[2] = s3 / g.stride[2];
     const int t0 = tc * TL;
     const R kappa = R(kappa_d), ccoef = R(ccoef_d);
-    // carried backward t-hop of the current slice: B = U_t(x-t)^dag P psi(x-t)
+    // carried backward t-hop of the current slice: B = U_t(x-t)^dag P psi(x-t).
+    // With FP16 pair products (kHP) it is formed from the integer components
+    // like every other hop and carries the neighbour's site scale Bc.
+    constexpr bool kHP = kHalfProducts<In, Lk> && !GP;
+    using LOp = LinkOp<Lk, R, kHP>;
     cx<R> B[NR][6];
+    R Bc[NR];
     {
         x[3] = t0;
         const int s = s3 + V3 * t0;
         bool w;
         const int tm = nbr_bwd(s, x, 3, g, w);
         const bool ng = w && g.phase_neg[3];
-        cx<R> u[9];
-        U.template ld9<R>(3, tm, u);
+        LOp u;
+        u.load(U, 3, tm);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             cx<R> psi[12], h[6];
-            in.ld12(rr[r], tm, psi);
+            if constexpr (kHP)
+                in.ld12s(rr[r], tm, psi, Bc[r]);
+            else
+                in.ld12(rr[r], tm, psi);
             if (GP) {
                 if (w && !g.phase_one[3]) phase_spinor(psi, g.iph_re[3], g.iph_im[3]);
             }
@@ -473,7 +483,7 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
                 else
                     neg_uniform(h, ng);
             }
-            matvec_bwd(u, h, B[r]);
+            u.bwd(h, B[r]);
         }
     }
     // FP64 (memory-bound): the t+1 spinor loaded for the forward t-hop is
@@ -508,17 +518,17 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
         hop_half<2, FPM, true, NR, GP>(acc, in, U, g, s, x, rr);
         hop_half<2, -FPM, false, NR, GP>(acc, in, U, g, s, x, rr);
         // t forward from memory, t backward from the carried B (reference order: forward, then backward)
-        cx<R> ut[9];
+        LOp ut;
         {
             bool w;
             const int tp = nbr_fwd(s, x, 3, g, w);
             const bool ng = w && g.phase_neg[3];
 #if LMX_NT_UT
             if constexpr (std::is_same<Lk, LinkF64>::value)
-                U.template ld9_nt<R>(3, s, ut);  // U_t(x) is used once: its backward use is the carried B
+                U.template ld9_nt<R>(3, s, ut.u);  // U_t(x) is used once: its backward use is the carried B
             else
 #endif
-                U.template ld9<R>(3, s, ut);
+                ut.load(U, 3, s);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 cx<R> psi[12], h[6], o[6];
@@ -539,9 +549,12 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
                 else
                     neg_uniform(h, ng);
             }
-                matvec_fwd(ut, h, o);
+                ut.fwd(h, o);
                 reconstruct_add_s<FPM, 3>(acc[r], o, c);
-                reconstruct_add<-FPM, 3>(acc[r], B[r]);
+                if constexpr (kHP)
+                    reconstruct_add_s<-FPM, 3>(acc[r], B[r], Bc[r]);
+                else
+                    reconstruct_add<-FPM, 3>(acc[r], B[r]);
             }
         }
         double red[NR];
@@ -553,12 +566,19 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
             if (kCarry && stp > 0) {
 #pragma unroll
                 for (int k = 0; k < 12; ++k) p[k] = mk<R>(cur[r][k].re * curc[r], cur[r][k].im * curc[r]);
+            } else if constexpr (kHP) {
+                R c;
+                in.ld12s(rr[r], s, p, c);
+                half_project<-FPM, 3>(p, h);
+                Bc[r] = c;
+#pragma unroll
+                for (int k = 0; k < 12; ++k) p[k] = mk<R>(p[k].re * c, p[k].im * c);
             } else {
                 in.ld12(rr[r], s, p);
             }
             red[r] = finish_site<MODE, EPI>(out, p, F, aux, ex, rr[r], s, g, kappa, ccoef, acc[r], r < nr);
-            half_project<-FPM, 3>(p, h);
-            matvec_bwd(ut, h, B[r]);
+            if constexpr (!kHP) half_project<-FPM, 3>(p, h);
+            ut.bwd(h, B[r]);
         }
         if constexpr (EPI != EPI_NONE) {
             // wave sums of this step; the 4-wave combination waits for the end
@@ -597,14 +617,14 @@ __global__ void __launch_bounds__(256) k_stream(Out out, In in, Lk U, Cl F, Aux 
 // hop order are k_stream's, so its results and block partials are
 // bit-identical to k_stream at one RHS per work item.
 // ---------------------------------------------------------------------------
-template <int MU, int PM, bool FWD, typename R>
-__device__ __forceinline__ void hop_given(cx<R> (&acc)[12], cx<R> (&psi)[12], R c, const cx<R> (&u)[9]) {
+template <int MU, int PM, bool FWD, typename R, class LOp>
+__device__ __forceinline__ void hop_given(cx<R> (&acc)[12], cx<R> (&psi)[12], R c, const LOp& u) {
     cx<R> h[6], o[6];
     half_project<PM, MU>(psi, h);
     if (FWD)
-        matvec_fwd(u, h, o);
+        u.fwd(h, o);
     else
-        matvec_bwd(u, h, o);
+        u.bwd(h, o);
     reconstruct_add_s<PM, MU>(acc, o, c);
     pin_acc(acc);  // as hop_half: the next hop's loads are not hoisted above this one
 }
@@ -616,10 +636,11 @@ __device__ __forceinline__ void hop_given(cx<R> (&acc)[12], cx<R> (&psi)[12], R 
 template <int MU, int FPM, int WHERE, class In, class Lk, typename R>
 __device__ __forceinline__ void hop_pair_lds(cx<R> (&acc)[12], const In& in, const typename In::Slot& sc, const Lk& U,
                                              const Geom& g, int r, int s, const int x[4], int base) {
-    cx<R> u[9], psi[12];
+    LinkOp<Lk, R, kHalfProducts<In, Lk>> u;
+    cx<R> psi[12];
     R c;
     bool w;
-    U.template ld9<R>(MU, s, u);
+    u.load(U, MU, s);
     const int nf = nbr_fwd(s, x, MU, g, w);
     if (WHERE == 1 || (WHERE == 0 && unsigned(nf - base) < 256u))
         In::lds_ld12s(sc, nf - base, psi, c);
@@ -627,7 +648,7 @@ __device__ __forceinline__ void hop_pair_lds(cx<R> (&acc)[12], const In& in, con
         in.ld12s(r, nf, psi, c);
     hop_given<MU, FPM, true>(acc, psi, c, u);
     const int nb = nbr_bwd(s, x, MU, g, w);
-    U.template ld9<R>(MU, nb, u);
+    u.load(U, MU, nb);
     if (WHERE == 1 || (WHERE == 0 && unsigned(nb - base) < 256u))
         In::lds_ld12s(sc, nb - base, psi, c);
     else
@@ -666,22 +687,30 @@ __global__ void __launch_bounds__(256) k_lds(Out out, In in, Lk U, Cl F, Aux aux
     const int t0 = tc * TL;
     const R kappa = R(kappa_d), ccoef = R(ccoef_d);
     // carried backward t-hop of the current slice: B = U_t(x-t)^dag P psi(x-t)
+    // (with FP16 pair products from the integer components, scale Bc, as k_stream)
+    constexpr bool kHP = kHalfProducts<In, Lk>;
+    using LOp = LinkOp<Lk, R, kHP>;
     cx<R> B[6];
+    R Bc = R(1);
     {
         x[3] = t0;
         const int s = s3 + V3 * t0;
         bool w;
         const int tm = nbr_bwd(s, x, 3, g, w);
         const bool ng = w && g.phase_neg[3];
-        cx<R> u[9], psi[12], h[6];
-        U.template ld9<R>(3, tm, u);
-        in.ld12(r, tm, psi);
+        LOp u;
+        cx<R> psi[12], h[6];
+        u.load(U, 3, tm);
+        if constexpr (kHP)
+            in.ld12s(r, tm, psi, Bc);
+        else
+            in.ld12(r, tm, psi);
         half_project<-FPM, 3>(psi, h);
         if constexpr (MODE == MODE_CLOVER)
             neg_select(h, ng);
         else
             neg_uniform(h, ng);
-        matvec_bwd(u, h, B);
+        u.bwd(h, B);
     }
     in.stage(r, s3 + V3 * t0, slot[0], tid);
     for (int stp = 0; stp < TL; ++stp) {
@@ -703,8 +732,8 @@ __global__ void __launch_bounds__(256) k_lds(Out out, In in, Lk U, Cl F, Aux aux
         hop_pair_lds<1, FPM, 0>(acc, in, sc, U, g, r, s, x, base);
         hop_pair_lds<2, FPM, ROWS ? 2 : 0>(acc, in, sc, U, g, r, s, x, base);
         // t forward from the staged slice t+1, t backward from the carried B
-        cx<R> ut[9];
-        U.template ld9<R>(3, s, ut);
+        LOp ut;
+        ut.load(U, 3, s);
         {
             const bool ng = wt && g.phase_neg[3];
             cx<R> psi[12], h[6], o[6];
@@ -715,9 +744,12 @@ __global__ void __launch_bounds__(256) k_lds(Out out, In in, Lk U, Cl F, Aux aux
                 neg_select(h, ng);
             else
                 neg_uniform(h, ng);
-            matvec_fwd(ut, h, o);
+            ut.fwd(h, o);
             reconstruct_add_s<FPM, 3>(acc, o, c);
-            reconstruct_add<-FPM, 3>(acc, B);
+            if constexpr (kHP)
+                reconstruct_add_s<-FPM, 3>(acc, B, Bc);
+            else
+                reconstruct_add<-FPM, 3>(acc, B);
             pin_acc(acc);
         }
         double red;
@@ -725,11 +757,15 @@ __global__ void __launch_bounds__(256) k_lds(Out out, In in, Lk U, Cl F, Aux aux
             cx<R> p[12], h[6];
             R c;
             In::lds_ld12s(sc, tid, p, c);
+            // the next slice's carried hop first (U_t(x) is then dead in the epilogue)
+            if constexpr (kHP) {
+                half_project<-FPM, 3>(p, h);
+                Bc = c;
+            }
 #pragma unroll
             for (int k = 0; k < 12; ++k) p[k] = mk<R>(p[k].re * c, p[k].im * c);
-            // the next slice's carried hop first (U_t(x) is then dead in the epilogue)
-            half_project<-FPM, 3>(p, h);
-            matvec_bwd(ut, h, B);
+            if constexpr (!kHP) half_project<-FPM, 3>(p, h);
+            ut.bwd(h, B);
             pin6f(B);
             red = finish_site<MODE, EPI, true>(out, p, F, aux, ex, r, s, g, kappa, ccoef, acc, true);
         }
@@ -782,29 +818,26 @@ __device__ __forceinline__ void hop_words(cx<float> (&acc)[12], const In& in, co
     bool w;
     const int t = FWD ? nbr_fwd(s, x, MU, g, w) : nbr_bwd(s, x, MU, g, w);
     const bool ng = w && g.phase_neg[MU];
-    cx<float> u[9], psi[12], h[6], o[6];
+    cx<float> psi[12], h[6], o[6];
     // the link words "change" here, after the previous hop's register pin:
-    // their FP16 -> FP32 conversions stay with this hop instead of all eight
-    // links' conversions being scheduled at the top of the RHS loop
+    // their use (FP16 -> FP32 conversions, or the pair products' operands)
+    // stays with this hop instead of being scheduled at the top of the RHS loop
     int lw9[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         lw9[k] = lw.w[k];
         asm volatile("" : "+v"(lw9[k]));
     }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const float2 v = h2f(lw9[k]);
-        u[k] = mk<float>(v.x, v.y);
-    }
+    LinkOp<LinkF16, float, kHalfProducts<In, LinkF16>> u;
+    u.from_words(lw9);
     float c;
     in.ld12s(r, t, psi, c);
     half_project<PM, MU>(psi, h);
     if (MU == 3) neg_select(h, ng);  // low-precision launches: only t can wrap with a sign
     if (FWD)
-        matvec_fwd(u, h, o);
+        u.fwd(h, o);
     else
-        matvec_bwd(u, h, o);
+        u.bwd(h, o);
     reconstruct_add_s<PM, MU>(acc, o, c);
     pin_acc(acc);
 }
