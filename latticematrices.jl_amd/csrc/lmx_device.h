@@ -238,6 +238,10 @@ __device__ __forceinline__ float4 ldg_stream(const float4* base, int s) {
     const lmx_fv4 v = ldg_raw<float4, lmx_fv4, LMX_NT_STREAM != 0>(base, s);
     return make_float4(v.x, v.y, v.z, v.w);
 }
+__device__ __forceinline__ double2 ldg_stream(const double2* base, int s) {
+    const lmx_dv2 v = ldg_raw<double2, lmx_dv2, LMX_NT_STREAM != 0>(base, s);
+    return make_double2(v.x, v.y);
+}
 
 struct SpF64In {
     using R = double;
@@ -259,6 +263,15 @@ struct SpF64In {
     __device__ __forceinline__ void ld12(int r, int s, cx<double>* z) const {
         double c;
         ld12s(r, s, z, c);
+    }
+    // streamed form (a field read once, at the site itself)
+    __device__ __forceinline__ void ld12_stream(int r, int s, cx<double>* z) const {
+        const double2* q = p + (size_t)r * 12 * V;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            double2 v = ldg_stream(q + (size_t)k * V, s);
+            z[k] = mk<double>(v.x, v.y);
+        }
     }
 };
 struct SpF64Out {

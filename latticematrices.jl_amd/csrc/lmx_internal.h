@@ -111,6 +111,15 @@ struct Epilogue {
 
 int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
                       double2* out, const double2* in, int nrhs, int flags, const Epilogue& epi);
+// FP64 D^dagger of the mixed solver's reliable update with the true residual
+// fused: r = b - D^dagger in, written in the inner format (spinor_prec LMX_H16
+// with its scales r_norm, or LMX_F32) for the RHS not retired (done) -- or, if
+// force, retired with fold set -- and |r|^2 as block partials, row r at
+// partials + r * pstride; gate as for Epilogue
+int launch_wilson_f64_true(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
+                           const double2* in, const double2* b, void* r, float* r_norm, int spinor_prec,
+                           const int* done, const int* fold, int dstride, int force, double* partials, long pstride,
+                           int nrhs, const int* gate);
 // mul! on the reference's halo-padded AoS spinor arrays (halo ignored, core written)
 int launch_wilson_padded(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
                          double2* C, const double2* psi, int nw, int nrhs, int flags);

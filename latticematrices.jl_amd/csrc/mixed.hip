@@ -339,37 +339,6 @@ __global__ void __launch_bounds__(256) k_mx_accum(double2* __restrict__ x, float
         accum_site(x, xlo, ring, V, rhs[i], mask, o, o4, i, s);
 }
 
-// reliable update, part 2: r = b - A x (A x in ax64), r_lo = r, |r|^2 in FP64
-template <class SO>
-__global__ void __launch_bounds__(256) k_mx_true_res(const double2* __restrict__ b, const double2* __restrict__ ax64, SO rlo,
-                                                     int V, double* partials, const MixRhs* rhs,
-                                                     const MixBatch* batch, int force) {
-    const int i = blockIdx.y;
-    if (!update_runs(rhs[i], batch, force)) return;
-    static_assert(kRedBlock == 256, "one partial per 256-site block");
-    const int nb = (V + 255) >> 8;
-    // grid-stride over the 256-site blocks (a small grid, cheap when gated
-    // off); one partial per block, as from a one-block-per-256-sites launch
-    for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
-        const int s = blk * 256 + threadIdx.x;
-        double v = 0.0;
-        if (s < V) {
-            const size_t o = (size_t)i * 12 * V;
-            cx<float> f[12];
-#pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                const size_t e = o + (size_t)k * V + s;
-                double2 bv = b[e], av = ax64[e];
-                double2 rv = make_double2(bv.x - av.x, bv.y - av.y);
-                f[k] = mk<float>((float)rv.x, (float)rv.y);
-                v += rv.x * rv.x + rv.y * rv.y;
-            }
-            rlo.st12(i, s, f);
-        }
-        row_partials(v, 0.0, partials, nb, blk);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Host driver
 // ---------------------------------------------------------------------------
@@ -487,12 +456,18 @@ static int apply_ddag_xr(MixRun& R, int e) {
 // off; they stride over the lattice with at most this many workgroups per RHS.
 constexpr unsigned kUpdateBlocks = 256;
 
-template <class SO>
-static void enqueue_true_res(MixRun& R, dim3 grid, int force) {
-    const int V = (int)R.ctx->V;
-    grid.x = std::min(grid.x, kUpdateBlocks);
-    hipLaunchKernelGGL((k_mx_true_res<SO>), grid, dim3(256), 0, R.ctx->stream, R.b, R.w.r64, out_of<SO>(R.w.r, V), V,
-                       R.w.partials, R.w.rhs, R.w.batch, force);
+// reliable update, parts 2 and 3: t64 = D x, then the FP64 D^dagger with the
+// true residual fused (r = b - D^dagger D x in the inner format, |r|^2 in
+// FP64 block partials); gated like part 1, or forced on the final pass
+static int enqueue_true_ddagd(MixRun& R, const double2* x, const int* gate, int force) {
+    lmx_wilson* op = R.op;
+    const Epilogue e{0, nullptr, nullptr, nullptr, nullptr, gate};
+    LMX_TRY(launch_wilson_f64(R.ctx, op->links, op->clover64, op->kappa, op->csw, R.w.t64, x, R.nrhs, 0, e));
+    const int* done = (const int*)((const char*)R.w.rhs + offsetof(MixRhs, done));
+    const int* fold = (const int*)((const char*)R.w.rhs + offsetof(MixRhs, fold));
+    return launch_wilson_f64_true(R.ctx, op->links, op->clover64, op->kappa, op->csw, R.w.t64, R.b, R.w.r.p, R.w.r.n,
+                                  R.spinor_prec, done, fold, (int)(sizeof(MixRhs) / sizeof(int)), force,
+                                  R.w.partials, 2L * nblocks(R.ctx->V), R.nrhs, gate);
 }
 template <class SO>
 static void enqueue_init(MixRun& R, dim3 grid) {
@@ -521,7 +496,6 @@ static void enqueue_accum(MixRun& R, dim3 grid, int force) {
 // One iteration with phase e (iteration k has e = k % kXBatch).
 static int enqueue_iteration(MixRun& R, int e) {
     lmx_context* c = R.ctx;
-    const bool h16 = R.spinor_prec == LMX_H16;
     dim3 grid(nblocks(c->V), R.nrhs), blk(256);
     const int nb = nblocks(c->V);
     MixRhs* rhs = R.w.rhs;
@@ -531,11 +505,7 @@ static int enqueue_iteration(MixRun& R, int e) {
     LMX_TRY(apply_ddag_xr(R, e));
     hipLaunchKernelGGL(k_mx_fin<MX_RR>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, e);
     enqueue_accum(R, grid, 0);
-    LMX_TRY(apply_ddagd64(R, R.w.r64, R.x, R.w.t64, &bt->do_update));
-    if (h16)
-        enqueue_true_res<SpH16Out>(R, grid, 0);
-    else
-        enqueue_true_res<SpF32Out>(R, grid, 0);
+    LMX_TRY(enqueue_true_ddagd(R, R.x, &bt->do_update, 0));
     hipLaunchKernelGGL(k_mx_fin<MX_TRUE>, dim3(R.nrhs), blk, 0, c->stream, R.w.partials, nb, rhs, bt, 0.0, 0.0, 0);
     LMX_HIP(hipGetLastError());
     return LMX_OK;
@@ -659,11 +629,7 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     // solution in and report the true residual; the ring slots marked in
     // pmask hold the still deferred alpha p
     enqueue_accum(R, grid, 1);
-    LMX_TRY(apply_ddagd64(R, R.w.r64, x, R.w.t64, nullptr));
-    if (R.spinor_prec == LMX_H16)
-        enqueue_true_res<SpH16Out>(R, grid, 1);
-    else
-        enqueue_true_res<SpF32Out>(R, grid, 1);
+    LMX_TRY(enqueue_true_ddagd(R, x, nullptr, 1));
     hipLaunchKernelGGL(k_mx_fin<MX_TRUE>, dim3(nrhs), blk, 0, c->stream, R.w.partials, nblocks(c->V), R.w.rhs,
                        R.w.batch, 0.0, 0.0, 1);
     LMX_HIP(hipGetLastError());

@@ -46,6 +46,39 @@ int launch_wilson_f64(lmx_context* ctx, const double2* U, const double2* clover,
         ctx, o, i, l, F, i, none, kappa, ccoef, nrhs, nrhs, flags, clover != nullptr, epi, 256);
 }
 
+template <class SO>
+static int wilson_f64_true(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
+                           const double2* in, const double2* b, SO rout, const int* done, const int* fold,
+                           int dstride, int force, double* partials, long pstride, int nrhs, const int* gate) {
+    const double ccoef = (-kappa) * csw;
+    const int V = (int)ctx->V;
+    SpF64Out o{nullptr, V};  // D^dagger in itself is not stored
+    SpF64In i{in, V};
+    LinkF64 l{U, V};
+    F64Clover F{clover};
+    EpiArgs epi{partials, pstride, nullptr, gate};
+    TrueEx<SO> ex{SpF64In{b, V}, rout, done, fold, dstride, force};
+    if (nrhs >= 2 && clover == nullptr)  // as launch_wilson_f64: two Wilson RHS per work item
+        return dispatch<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, TrueEx<SO>, 2, EPI_TRUE, true, true>(
+            ctx, o, i, l, F, i, ex, kappa, ccoef, (nrhs + 1) / 2, nrhs, LMX_DAGGER, false, epi, 256);
+    return dispatch<SpF64In, LinkF64, SpF64Out, F64Clover, SpF64In, TrueEx<SO>, 1, EPI_TRUE, true, true>(
+        ctx, o, i, l, F, i, ex, kappa, ccoef, nrhs, nrhs, LMX_DAGGER, clover != nullptr, epi, 256);
+}
+
+int launch_wilson_f64_true(lmx_context* ctx, const double2* U, const double2* clover, double kappa, double csw,
+                           const double2* in, const double2* b, void* r, float* r_norm, int spinor_prec,
+                           const int* done, const int* fold, int dstride, int force, double* partials, long pstride,
+                           int nrhs, const int* gate) {
+    if (ctx->V == 0 || nrhs == 0) return LMX_OK;
+    LMX_CHECK(ctx->nc == 3, LMX_ERR_UNSUPPORTED, "the fused true residual is built for NC = 3");
+    const int V = (int)ctx->V;
+    if (spinor_prec == LMX_H16)
+        return wilson_f64_true(ctx, U, clover, kappa, csw, in, b, SpH16Out::at(r, r_norm, V, 0), done, fold, dstride,
+                               force, partials, pstride, nrhs, gate);
+    return wilson_f64_true(ctx, U, clover, kappa, csw, in, b, SpF32Out::at(r, r_norm, V, 0), done, fold, dstride,
+                           force, partials, pstride, nrhs, gate);
+}
+
 // ---------------------------------------------------------------------------
 // Low-precision variants for the mixed / multi-RHS solver: spinors FP32 or
 // 16-bit (fixed point with a per-site FP32 scale), links FP32 or FP16, FP32

@@ -195,7 +195,7 @@ enum { MODE_WILSON = 0, MODE_DONLY = 1, MODE_CLOVER = 2 };
 //                  EPI_TP    the stencil input is r: out = t = D r + beta t in
 //                            place (= D p by linearity), p = r + beta p_prev
 //                            written at the site, and |t|^2 (mixed solver, on D).
-enum { EPI_NONE = 0, EPI_PAP = 1, EPI_XR = 3, EPI_TP = 4 };
+enum { EPI_NONE = 0, EPI_PAP = 1, EPI_XR = 3, EPI_TP = 4, EPI_TRUE = 5 };
 
 struct NoEx {};
 // Extra fields of EPI_XR, all offset to the launch's first RHS.  The x
@@ -238,6 +238,22 @@ struct TpEx {
     int astride;
     const int* done;      // retired RHS (p left as it is) at done[r * dstride]
     int dstride;
+};
+
+// EPI_TRUE (FP64 D^dagger of a reliable update, the operand D x): the true
+// residual r = b - D^dagger D x at the site in FP64, written to the inner
+// solver's r (16-bit or FP32) instead of storing D^dagger D x, with |r|^2 as
+// the site's contribution (FP64, the unfused kernel's sum order).  An RHS
+// takes part unless retired (done), or, on the forced final pass, retired
+// with its fold outstanding; the batch-wide request is the launch's gate.
+template <class Out>
+struct TrueEx {
+    SpF64In b;           // right-hand sides
+    Out rout;            // r, in the inner format
+    const int* done;     // done[r * dstride]
+    const int* fold;     // fold[r * dstride]
+    int dstride;
+    int force;
 };
 
 // Finish one site from its own spinor psi: out = psi - kappa*hop (+ clover),
@@ -336,6 +352,21 @@ __device__ __forceinline__ double finish_site(const Out& out, const cx<R> (&psi)
             ex.xout.st12(r, s, xv);
         }
         return red;
+    } else if constexpr (EPI == EPI_TRUE) {
+        static_assert(std::is_same<R, double>::value, "the true residual is an FP64 epilogue");
+        cx<double> bv[12];
+        ex.b.ld12_stream(r, s, bv);
+        cx<float> f[12];
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const double re = bv[k].re - res[k].re, im = bv[k].im - res[k].im;
+            f[k] = mk<float>((float)re, (float)im);
+            v += re * re + im * im;
+        }
+        const int d = ex.done[(long)r * ex.dstride];
+        if (store && (d == 0 || (ex.force && ex.fold[(long)r * ex.dstride] != 0))) ex.rout.st12(r, s, f);
+        return v;
     } else {
         if (store) out.st12(r, s, res);
     }
@@ -1243,7 +1274,7 @@ int dispatch(lmx_context* ctx, Out out, In in, Lk U, Cl F, Aux aux, const Ex& ex
             hipLaunchKernelGGL((k_site<In, Lk, Out, Cl, Aux, Ex, FPM, MODE, NR, EPI, false>), dim3(nb_site),   \
                                dim3(256), 0, st, out, in, U, F, aux, ex, ctx->g, kappa, ccoef, ngroups, nrhs, epi); \
     } while (0)
-    if constexpr (EPI == EPI_PAP || EPI == EPI_XR) {
+    if constexpr (EPI == EPI_PAP || EPI == EPI_XR || EPI == EPI_TRUE) {
         if (clover)
             LMX_GO(1, MODE_CLOVER);
         else
