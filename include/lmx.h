@@ -52,7 +52,8 @@ enum {
 
 /* ---- precisions of stored fields --------------------------------------- */
 /* LMX_H16: spinors as 16-bit fixed point with one FP32 scale per site
- * (value = q * scale, scale = largest |component| / 32767), links FP16; the
+ * (value = q * scale, scale = largest |component| / 32752, so that the sum
+ * of two components stays within the FP16 range), links FP16; the
  * mixed solver's inner vectors.  Spinor layout int4[nrhs][3][V] followed by
  * float[nrhs][V]. */
 enum { LMX_F64 = 0, LMX_F32 = 1, LMX_F16 = 2, LMX_H16 = 3 };
