@@ -596,6 +596,7 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
         }
         c->stream = saved;
         (void)hipStreamDestroy(cap);
+        if ((rc != LMX_OK || e != hipSuccess) && graph) (void)hipGraphDestroy(graph);
         LMX_TRY(rc);
         LMX_HIP(e);
         LMX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
