@@ -34,6 +34,28 @@ def oracle_check(U, L, kappa, b, x, rtol, csw=0.0):
     return out
 
 
+def test_h16_projection_at_full_scale(gpu):
+    # Every component of every site equal (RHS 0: 1 + i; RHS 1: 1 - i): the
+    # 16-bit fixed-point values all sit at the format's maximum, 32752, and
+    # the projected half-spinors, sums of two components, reach 2 x 32752 =
+    # 65504, the largest finite FP16 value, in the FP16 pair products of the
+    # link multiplication.  With a maximum of 32767 (65534) they would round
+    # to infinity and the solve would break down at its first iteration.
+    L, kappa, rtol = (8, 8, 4, 8), 0.124, 1e-10
+    U, b, lat, Ug, D = setup(L, "weak", 341, kappa, nrhs=2)
+    b = np.empty_like(b)
+    b[0] = 1 + 1j
+    b[1] = 1 - 1j
+    print(f"h16 at full scale L={L}", flush=True)
+    x = lat.spinor(2)
+    res = lmx.cg_batched_(x, D, lat.spinor_from_numpy(b), precision="h16", rtol=rtol, use_graph=False)
+    checks = oracle_check(U, L, kappa, b, x.to_numpy(), rtol)
+    for r, (tr, err) in zip(res, checks):
+        print(f"  iters={r.iterations} reason={r.reason} true_rel={tr:.3e} err_vs_fp64={err:.3e}", flush=True)
+        assert r.converged and r.reason == "converged"
+        assert tr <= rtol * 1.05 and err <= 1e-7
+
+
 @pytest.mark.parametrize("precision", ["f32", "f16", "h16"])
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_mixed_batched_converges_to_fp64_true_residual(gpu, precision, use_graph):
