@@ -421,7 +421,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"h16": "16-bit fixed-point inner spinors, f16 links, f32 math, f64 reliable updates",
+        "dtype": {"h16": "16-bit fixed-point inner spinors, f16 links, link products on f16 operands with f32 accumulation, f32 math otherwise, f64 reliable updates",
                   "f16": "f16 links, f32 inner spinors and math, f64 reliable updates",
                   "f32": "f32 inner links, spinors and math, f64 reliable updates"}[args.precision],
         "data": "synthetic (seeded weak-field SU(3) links eps=0.2; the reference benchmark's Gaussian sources, "

@@ -311,7 +311,9 @@ int lmx_cg_generic(lmx_context* ctx, lmx_apply_fn A, void* A_user, void* x, cons
  * fields, x is the initial guess).  params->precision selects plain FP64 CG
  * per RHS (LMX_F64) or the mixed solver: FP32 (LMX_F32), FP16-stored links
  * with FP32 spinors (LMX_F16), or FP16 links with 16-bit spinors (LMX_H16)
- * inner CG (FP32 arithmetic) with FP64 reliable updates, FP64
+ * inner CG (FP32 arithmetic; with LMX_H16 the link products multiply FP16
+ * link entries by the projected half-spinor rounded to FP16, accumulating in
+ * FP32) with FP64 reliable updates, FP64
  * reductions and FP64 solution accumulation.  Convergence is on the FP64
  * true residual |b - A x| <= max(atol, rtol |b|) per RHS.  The workspace is
  * caller-owned (size from lmx_cg_batched_workspace).  results[nrhs]. */
