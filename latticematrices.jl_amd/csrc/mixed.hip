@@ -563,7 +563,6 @@ int cg_batched_impl(lmx_wilson* op, double2* x, const double2* b, int nrhs, void
     R.b = b;
     R.nrhs = nrhs;
     R.rhs_block = params->rhs_block > 0 ? params->rhs_block : default_rhs_block(op, params->precision, nrhs);
-    const int V = (int)c->V;
     dim3 grid(nblocks(c->V), nrhs), blk(256);
     const int rmax = params->reliable_maxiter > 0 ? params->reliable_maxiter : 100;
     hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(64), 0, c->stream, R.w.rhs, R.w.batch, nrhs, params->reliable_delta,
